@@ -1,0 +1,188 @@
+/* ssim2hip -- ssimulacra2 on AMD Instinct MI355X (gfx950), C ABI.
+ *
+ * Drop-in boundary for the ssimulacra2 engine of imazen/turbo-metrics
+ * (crates/ssimulacra2-cuda). Each entry point below names the Rust method it
+ * replaces as `file:line`; paths are relative to the turbo-metrics checkout.
+ * Images are device buffers described by plain pointers and sizes; streams are
+ * hipStream_t passed as an opaque pointer (NULL = the default stream). Every
+ * call returns an ssim2_status; nothing aborts the process.
+ *
+ * Score semantics follow the reference's CPU ssimulacra2
+ * (crates/ssimulacra2-cuda/examples/cpu.rs): the same f32 arithmetic, the same
+ * scale count (the first scale smaller than 8 px is still scored) and the same
+ * 108-weight table. Images smaller than 8x8 are rejected with
+ * SSIM2_INVALID_SIZE (the documented error of cpu.rs:335-341).
+ */
+#ifndef SSIM2HIP_H
+#define SSIM2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SSIM2_OK = 0,
+  SSIM2_INVALID_SIZE = 1,      /* width or height < 8, or beyond the supported maximum */
+  SSIM2_DIM_MISMATCH = 2,      /* image size differs from the context's */
+  SSIM2_NO_REFERENCE = 3,      /* cached-reference call without set_reference */
+  SSIM2_OOM = 4,               /* device allocation failed or pre-flight budget exceeded */
+  SSIM2_BATCH_SIZE = 5,        /* batch larger than the context was created for */
+  SSIM2_ADMISSION_REFUSED = 6, /* serving: request does not fit the HBM budget */
+  SSIM2_HIP_ERROR = 7,         /* a HIP runtime call failed (see ssim2_last_error) */
+  SSIM2_INVALID_ARGUMENT = 8,  /* null pointer, bad pitch, bad option value */
+  SSIM2_NOT_READY = 9          /* serving: no finished result to return yet */
+} ssim2_status;
+
+/* Opaque hipStream_t. NULL selects the default (null) stream. */
+typedef void* ssim2_stream;
+
+/* Packed RGB device image: 3 samples per pixel, u8 or f32, rows `pitch_bytes`
+ * apart. Mirrors cudarse-npp Image<S, C<3>> (crates/cudarse/cudarse-npp/src/
+ * image/mod.rs:138-147). */
+typedef struct {
+  void* ptr;
+  size_t pitch_bytes;
+  uint32_t width;
+  uint32_t height;
+} ssim2_image;
+
+typedef struct ssim2_ctx ssim2_ctx;
+
+/* Context options. Zero-initialise, then set struct_size = sizeof(ssim2_options). */
+typedef struct {
+  uint32_t struct_size;
+  uint32_t reserved0;
+  /* Refuse creation when the estimated device footprint exceeds this many
+   * bytes (0 = use the device's free memory, as the reference's pre-flight
+   * check does, crates/ssimulacra2-cuda/src/lib.rs:88-112). */
+  uint64_t memory_budget;
+} ssim2_options;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Ssimulacra2::new (crates/ssimulacra2-cuda/src/lib.rs:80-197). Unlike the
+ * reference, the context is not bound to particular input buffers: every
+ * compute call names its images, so buffers may change between calls. */
+ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* options,
+                          ssim2_ctx** out);
+
+/* Drop for Ssimulacra2 (lib.rs:1195-1202): waits for the context's work,
+ * frees everything, never aborts. NULL is accepted. */
+ssim2_status ssim2_destroy(ssim2_ctx* ctx);
+
+/* Ssimulacra2::mem_usage (lib.rs:206-235): device bytes owned by the context. */
+ssim2_status ssim2_mem_usage(const ssim2_ctx* ctx, size_t* bytes);
+
+/* ---- scoring ------------------------------------------------------------ */
+
+/* Ssimulacra2::compute (lib.rs:419-424): enqueue the whole pipeline on
+ * `stream` for a pair of packed linear-RGB f32 images; returns at once.
+ * Fetch the score with ssim2_get_score. */
+ssim2_status ssim2_compute_linear_async(ssim2_ctx* ctx, const ssim2_image* ref_linear,
+                                        const ssim2_image* dis_linear, ssim2_stream stream);
+
+/* Ssimulacra2::get_score (lib.rs:426-429): waits for the last enqueued
+ * computation of this context and returns its score. Idempotent (the
+ * reference's version post-processes in place and must not be called twice). */
+ssim2_status ssim2_get_score(ssim2_ctx* ctx, double* score);
+
+/* Ssimulacra2::compute_sync (lib.rs:409-416). */
+ssim2_status ssim2_compute_linear_sync(ssim2_ctx* ctx, const ssim2_image* ref_linear,
+                                       const ssim2_image* dis_linear, ssim2_stream stream,
+                                       double* score);
+
+/* Ssimulacra2::compute_srgb_sync (lib.rs:391-404): sRGB u8 device images in.
+ * When ref_linear / dis_linear are non-NULL they receive the linear f32 copy
+ * (as the reference's srgb_to_linear step writes them); NULL skips that write. */
+ssim2_status ssim2_compute_srgb_sync(ssim2_ctx* ctx, const ssim2_image* ref_srgb,
+                                     const ssim2_image* dis_srgb, ssim2_image* ref_linear,
+                                     ssim2_image* dis_linear, ssim2_stream stream, double* score);
+
+/* Asynchronous form of ssim2_compute_srgb_sync without the linear copies:
+ * u8 in, score fetched with ssim2_get_score. The reference has no separate
+ * method; compute_srgb_sync (lib.rs:391-404) without its final sync is this. */
+ssim2_status ssim2_compute_srgb_async(ssim2_ctx* ctx, const ssim2_image* ref_srgb,
+                                      const ssim2_image* dis_srgb, ssim2_stream stream);
+
+/* Ssimulacra2::compute_from_cpu_srgb_sync (lib.rs:329-345): packed sRGB bytes
+ * in host memory (`row_bytes` apart, at least 3*width), uploaded into the
+ * context's own staging images. */
+ssim2_status ssim2_compute_from_host_srgb(ssim2_ctx* ctx, const uint8_t* ref, const uint8_t* dis,
+                                          size_t row_bytes, ssim2_stream stream, double* score);
+
+/* ---- cached reference ---------------------------------------------------- */
+
+/* Ssimulacra2::set_reference_linear (lib.rs:440-541): compute and keep the
+ * reference-side state (XYB pyramid, blurred ref^2 and ref) on the device. */
+ssim2_status ssim2_set_reference_linear(ssim2_ctx* ctx, const ssim2_image* ref_linear,
+                                        ssim2_stream stream);
+
+/* sRGB u8 form of ssim2_set_reference_linear, covering the reference's
+ * prepare_reference_from_cpu_srgb + set_reference_linear sequence
+ * (lib.rs:373-388, 440-541) for device-resident bytes. */
+ssim2_status ssim2_set_reference_srgb(ssim2_ctx* ctx, const ssim2_image* ref_srgb,
+                                      ssim2_stream stream);
+
+/* Ssimulacra2::compute_with_reference_linear (lib.rs:543-590): score a linear
+ * distorted image against the cached reference (SSIM2_NO_REFERENCE without
+ * one; the reference panics). `ref_linear` is accepted for signature parity
+ * and may be NULL: the cached state is used. */
+ssim2_status ssim2_compute_with_reference_linear(ssim2_ctx* ctx, const ssim2_image* ref_linear,
+                                                 const ssim2_image* dis_linear,
+                                                 ssim2_stream stream, double* score);
+
+/* sRGB u8 form (the reference's compute_from_cpu_dis_only_srgb_sync,
+ * lib.rs:355-371, for device-resident bytes). */
+ssim2_status ssim2_compute_with_reference_srgb(ssim2_ctx* ctx, const ssim2_image* dis_srgb,
+                                               ssim2_stream stream, double* score);
+
+/* Ssimulacra2::clear_reference / has_reference (lib.rs:592-602). */
+ssim2_status ssim2_clear_reference(ssim2_ctx* ctx);
+ssim2_status ssim2_has_reference(const ssim2_ctx* ctx, int* has_reference);
+
+/* ---- diagnostics --------------------------------------------------------- */
+
+/* The raw per-scale sums behind the last score: sums[scale*18 + c*6 + j],
+ * j = {ssim sum d, ssim sum d^4, artifact sum, artifact sum^4, detail sum,
+ * detail sum^4}; dims[2*scale] = width, dims[2*scale+1] = height. Waits for
+ * the last computation. Any pointer may be NULL. */
+ssim2_status ssim2_get_sums(ssim2_ctx* ctx, double* sums108, int* dims12, int* nscales);
+
+/* ---- utilities ------------------------------------------------------------ */
+
+/* Kernel::srgb_to_linear (crates/ssimulacra2-cuda/src/kernel.rs:36-57):
+ * packed u8 sRGB -> packed f32 linear through the reference's 256-entry table. */
+ssim2_status ssim2_srgb_to_linear(const ssim2_image* src_u8, ssim2_image* dst_f32,
+                                  ssim2_stream stream);
+
+/* Device image helpers, the C form of cudarse-npp Image::malloc /
+ * copy_from_cpu / copy_to_cpu (crates/cudarse/cudarse-npp/src/image/isu.rs:18-40,
+ * image/mod.rs:410-439). bytes_per_sample is 1 (u8) or 4 (f32). Upload and
+ * download are synchronous. */
+ssim2_status ssim2_image_alloc(uint32_t width, uint32_t height, uint32_t bytes_per_sample,
+                               ssim2_image* out);
+ssim2_status ssim2_image_free(ssim2_image* image);
+ssim2_status ssim2_image_upload(ssim2_image* image, const void* host, size_t host_row_bytes);
+ssim2_status ssim2_image_download(const ssim2_image* image, void* host, size_t host_row_bytes);
+
+/* Stream helpers (cudarse-driver CuStream::new / sync, stream.rs:46,107). */
+ssim2_status ssim2_stream_create(ssim2_stream* out);
+ssim2_status ssim2_stream_destroy(ssim2_stream stream);
+ssim2_status ssim2_stream_synchronize(ssim2_stream stream);
+
+/* Human-readable name of a status, and the last error message recorded on the
+ * calling thread ("" if none). */
+const char* ssim2_status_string(ssim2_status status);
+const char* ssim2_last_error(void);
+
+/* Library version as major*10000 + minor*100 + patch. */
+int ssim2_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSIM2HIP_H */

@@ -1,0 +1,506 @@
+// ssimulacra2 CPU oracle -- TEST INFRASTRUCTURE ONLY.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// this library, and only as the checker / the timed CPU baseline. The product
+// (turbo-metrics_amd/) never links or calls it.
+//
+// It restates the reference's in-tree CPU ssimulacra2,
+// crates/ssimulacra2-cuda/examples/cpu.rs, in three arithmetic modes:
+//
+//   MODE_F32_MIRROR (0)  the same f32/f64 mix and operation order as cpu.rs,
+//                        fused multiply-adds exactly where cpu.rs calls
+//                        mul_add, f64 accumulation per pixel. Built with
+//                        -ffp-contract=off so no other contraction happens.
+//   MODE_F64        (1)  every stage in double; the blur is the exact
+//                        zero-padded 9-tap FIR the recursive Gaussian equals
+//                        (double taps below). This is the FP64 oracle.
+//   MODE_F32_FIR    (2)  f32 everywhere like the mirror, but the blur is the
+//                        9-tap FIR in f32 (diagnostic: the spread between
+//                        blur evaluation orders).
+//
+// Parity pins (each checked in tests/):
+//   * LUT and WEIGHT tables: equal, bit for bit, to the tables extracted from
+//     cpu.rs:20-277 and cpu.rs:729-838 (tests/golden/reference_tables.json).
+//   * IIR constants: re-derived in oracle/derive_iir.py and equal to
+//     cpu.rs:931-948; the double IIR equals the 9-tap FIR used by MODE_F64.
+//   * XYB: closed form of cpu.rs:458-496, compared with an independent double
+//     evaluation; downscale against brute force on odd sizes.
+//   * whole pipeline: identical inputs score exactly 100 (cpu.rs:862-868),
+//     JPEG corpus scores of the published ssimulacra2 0.5.1 crate (loose),
+//     monotone in distortion strength, FP32 mirror vs FP64 within bounds.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "ref_tables.h"
+
+namespace {
+
+const float kLut[256] = ORACLE_SRGB8_LUT_INIT;
+const double kWeight[108] = ORACLE_WEIGHTS_INIT;
+
+// ---- XYB constants, cpu.rs:420-434 (f32, const-evaluated in f32) ----
+const float K_M02 = 0.078f;
+const float K_M00 = 0.30f;
+const float K_M01 = (1.0f - K_M02) - K_M00;
+const float K_M12 = 0.078f;
+const float K_M10 = 0.23f;
+const float K_M11 = (1.0f - K_M12) - K_M10;
+const float K_M20 = 0.24342269f;
+const float K_M21 = 0.20476745f;
+const float K_M22 = (1.0f - K_M20) - K_M21;
+const float K_B0 = 0.0037930734f;
+const float K_B0_ROOT = 0.1559542025327239180319220163705f;
+
+// ---- recursive Gaussian, cpu.rs:931-948 (values re-derived by derive_iir.py) ----
+const int RADIUS = 5;
+const float MUL_IN[3] = {0x1.c4fa84p-5f, -0x1.e1fd7ap-5f, 0x1.a8894ap-7f};
+const float MUL_PREV[3] = {0x1.e6f0e2p+0f, 0x1.2cf230p+0f, 0x1.1a6264p-53f};
+const float MUL_PREV2[3] = {-1.0f, -1.0f, -1.0f};
+const float VERT_MUL_IN[3] = {0x1.c4fa84p-5f, -0x1.e1fd7ap-5f, 0x1.a8894ap-7f};
+const float VERT_MUL_PREV[3] = {-0x1.e6f0e2p+0f, -0x1.2cf230p+0f, -0x1.1a6264p-53f};
+
+// Centre-first taps of the 9-tap FIR equal to the double IIR (derive_iir.fir_taps_f64).
+const double FIR64[5] = {0x1.0ef8d5c79cdacp-2, 0x1.b413e7d88b729p-3, 0x1.bfd672a600a7cp-4,
+                         0x1.2700c7f8017a7p-5, 0x1.347d70ed6fe2cp-7};
+
+const int NUM_SCALES = 6;
+
+// ------------------------------------------------------------------ f32 mirror
+
+// cpu.rs:458-496 px_linear_rgb_to_xyb (f32, FMA chain as written).
+inline void xyb_f32(float r, float g, float b, float* X, float* Y, float* B) {
+  float rg = std::fma(K_M00, r, std::fma(K_M01, g, std::fma(K_M02, b, K_B0)));
+  float gr = std::fma(K_M10, r, std::fma(K_M11, g, std::fma(K_M12, b, K_B0)));
+  float bb = std::fma(K_M20, r, std::fma(K_M21, g, std::fma(K_M22, b, K_B0)));
+  rg = std::cbrt(std::max(rg, 0.0f)) - K_B0_ROOT;
+  gr = std::cbrt(std::max(gr, 0.0f)) - K_B0_ROOT;
+  bb = std::cbrt(std::max(bb, 0.0f)) - K_B0_ROOT;
+  float x = 0.5f * (rg - gr);
+  float y = 0.5f * (rg + gr);
+  *X = std::fma(x, 14.0f, 0.42f);
+  *Y = y + 0.01f;
+  *B = (bb - y) + 0.55f;
+}
+
+// cpu.rs:545-579: ceil(in/2), 2x2 mean with coordinates clamped to in-1.
+template <typename T>
+void downscale_packed(const T* in, int w, int h, T* out, int ow, int oh) {
+  const T norm = T(1) / T(4);
+  for (int oy = 0; oy < oh; ++oy)
+    for (int ox = 0; ox < ow; ++ox)
+      for (int c = 0; c < 3; ++c) {
+        T sum = 0;
+        for (int iy = 0; iy < 2; ++iy)
+          for (int ix = 0; ix < 2; ++ix) {
+            int x = std::min(ox * 2 + ix, w - 1);
+            int y = std::min(oy * 2 + iy, h - 1);
+            sum += in[(size_t(y) * w + x) * 3 + c];
+          }
+        out[(size_t(oy) * ow + ox) * 3 + c] = sum * norm;
+      }
+}
+
+// cpu.rs:967-1022 horizontal_row (row form).
+void iir_row_f32(const float* in, float* out, int w) {
+  float prev[3] = {0, 0, 0}, prev2[3] = {0, 0, 0};
+  for (int n = -RADIUS + 1; n < w; ++n) {
+    int left = n - RADIUS - 1, right = n + RADIUS - 1;
+    float lv = left >= 0 ? in[left] : 0.0f;
+    float rv = right < w ? in[right] : 0.0f;
+    float sum = lv + rv;
+    float o[3];
+    for (int k = 0; k < 3; ++k) {
+      o[k] = sum * MUL_IN[k];
+      o[k] = std::fma(MUL_PREV2[k], prev2[k], o[k]);
+      prev2[k] = prev[k];
+      o[k] = std::fma(MUL_PREV[k], prev[k], o[k]);
+      prev[k] = o[k];
+    }
+    if (n >= 0) out[n] = (o[0] + o[1]) + o[2];
+  }
+}
+
+// cpu.rs:1054-1115 vertical_pass (column form), applied to every column.
+void iir_cols_f32(const float* in, float* out, int w, int h) {
+  std::vector<float> prev(3 * size_t(w), 0.0f), prev2(3 * size_t(w), 0.0f);
+  for (int n = -RADIUS + 1; n < h; ++n) {
+    int top = n - RADIUS - 1, bottom = n + RADIUS - 1;
+    const float* trow = top >= 0 ? in + size_t(top) * w : nullptr;
+    const float* brow = bottom < h ? in + size_t(bottom) * w : nullptr;
+    for (int i = 0; i < w; ++i) {
+      float sum = (trow ? trow[i] : 0.0f) + (brow ? brow[i] : 0.0f);
+      float o[3];
+      for (int k = 0; k < 3; ++k) {
+        float t = std::fma(prev[k * size_t(w) + i], VERT_MUL_PREV[k], prev2[k * size_t(w) + i]);
+        o[k] = std::fma(sum, VERT_MUL_IN[k], -t);
+      }
+      for (int k = 0; k < 3; ++k) {
+        prev2[k * size_t(w) + i] = prev[k * size_t(w) + i];
+        prev[k * size_t(w) + i] = o[k];
+      }
+      if (n >= 0) out[size_t(n) * w + i] = (o[0] + o[1]) + o[2];
+    }
+  }
+}
+
+// cpu.rs:913-918 blur_plane: rows then columns.
+void blur_iir_f32(const float* in, float* out, int w, int h, std::vector<float>& tmp) {
+  tmp.resize(size_t(w) * h);
+  for (int y = 0; y < h; ++y) iir_row_f32(in + size_t(y) * w, tmp.data() + size_t(y) * w, w);
+  iir_cols_f32(tmp.data(), out, w, h);
+}
+
+// 9-tap zero-padded FIR, rows then columns, outer taps first.
+template <typename T>
+void blur_fir(const T* in, T* out, int w, int h, std::vector<T>& tmp) {
+  tmp.resize(size_t(w) * h);
+  T taps[5];
+  for (int k = 0; k < 5; ++k) taps[k] = T(FIR64[k]);
+  auto at_row = [&](const T* row, int x) -> T { return (x >= 0 && x < w) ? row[x] : T(0); };
+  for (int y = 0; y < h; ++y) {
+    const T* row = in + size_t(y) * w;
+    T* o = tmp.data() + size_t(y) * w;
+    for (int x = 0; x < w; ++x) {
+      T acc = taps[4] * (at_row(row, x - 4) + at_row(row, x + 4));
+      for (int k = 3; k >= 1; --k) acc = std::fma(taps[k], at_row(row, x - k) + at_row(row, x + k), acc);
+      o[x] = std::fma(taps[0], row[x], acc);
+    }
+  }
+  auto at_col = [&](int x, int y) -> T { return (y >= 0 && y < h) ? tmp[size_t(y) * w + x] : T(0); };
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      T acc = taps[4] * (at_col(x, y - 4) + at_col(x, y + 4));
+      for (int k = 3; k >= 1; --k) acc = std::fma(taps[k], at_col(x, y - k) + at_col(x, y + k), acc);
+      out[size_t(y) * w + x] = std::fma(taps[0], tmp[size_t(y) * w + x], acc);
+    }
+}
+
+// Per-scale sums in the layout sums[c*6 + j], j = {ssim S1, ssim S4, art S1,
+// art S4, det S1, det S4}; S1 = sum of d, S4 = sum of d^4 (f64).
+//
+// cpu.rs:581-638 ssim_map (f32 formula, f64 "1 -" and accumulation).
+void ssim_sums_f32(int n, const float* m1, const float* m2, const float* s11, const float* s22,
+                   const float* s12, double* s1, double* s4) {
+  const float C2 = 0.0009f;
+  double a = 0, b = 0;
+  for (int i = 0; i < n; ++i) {
+    float mu1 = m1[i], mu2 = m2[i];
+    float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
+    float mu_diff = mu1 - mu2;
+    float num_m = std::fma(mu_diff, -mu_diff, 1.0f);
+    float num_s = std::fma(2.0f, s12[i] - mu12, C2);
+    float denom_s = ((s11[i] - mu11) + (s22[i] - mu22)) + C2;
+    double d = 1.0 - double((num_m * num_s) / denom_s);
+    d = std::max(d, 0.0);
+    a += d;
+    double d2 = d * d;
+    b += d2 * d2;
+  }
+  *s1 = a;
+  *s4 = b;
+}
+
+// cpu.rs:640-683 edge_diff_map (f32 differences, f64 ratio).
+void edge_sums_f32(int n, const float* i1, const float* m1, const float* i2, const float* m2,
+                   double* out4) {
+  double s[4] = {0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    double d1 = (1.0 + double(std::fabs(i2[i] - m2[i]))) / (1.0 + double(std::fabs(i1[i] - m1[i]))) - 1.0;
+    double art = std::max(d1, 0.0);
+    double a2 = art * art;
+    s[0] += art;
+    s[1] += a2 * a2;
+    double det = std::max(-d1, 0.0);
+    double e2 = det * det;
+    s[2] += det;
+    s[3] += e2 * e2;
+  }
+  for (int k = 0; k < 4; ++k) out4[k] = s[k];
+}
+
+// ------------------------------------------------------------------ f64 oracle
+
+inline void xyb_f64(double r, double g, double b, double* X, double* Y, double* B) {
+  const double m00 = 0.30, m02 = 0.078, m01 = 1.0 - m02 - m00;
+  const double m10 = 0.23, m12 = 0.078, m11 = 1.0 - m12 - m10;
+  const double m20 = 0.24342269, m21 = 0.20476745, m22 = 1.0 - m20 - m21;
+  const double bias = 0.0037930734;
+  const double root = 0.1559542025327239180319220163705;
+  double rg = m00 * r + m01 * g + m02 * b + bias;
+  double gr = m10 * r + m11 * g + m12 * b + bias;
+  double bb = m20 * r + m21 * g + m22 * b + bias;
+  rg = std::cbrt(std::max(rg, 0.0)) - root;
+  gr = std::cbrt(std::max(gr, 0.0)) - root;
+  bb = std::cbrt(std::max(bb, 0.0)) - root;
+  double x = 0.5 * (rg - gr), y = 0.5 * (rg + gr);
+  *X = x * 14.0 + 0.42;
+  *Y = y + 0.01;
+  *B = bb - y + 0.55;
+}
+
+void ssim_sums_f64(int n, const double* m1, const double* m2, const double* s11, const double* s22,
+                   const double* s12, double* s1, double* s4) {
+  const double C2 = 0.0009;
+  double a = 0, b = 0;
+  for (int i = 0; i < n; ++i) {
+    double md = m1[i] - m2[i];
+    double num_m = 1.0 - md * md;
+    double num_s = 2.0 * (s12[i] - m1[i] * m2[i]) + C2;
+    double den = (s11[i] - m1[i] * m1[i]) + (s22[i] - m2[i] * m2[i]) + C2;
+    double d = std::max(1.0 - num_m * num_s / den, 0.0);
+    a += d;
+    b += (d * d) * (d * d);
+  }
+  *s1 = a;
+  *s4 = b;
+}
+
+void edge_sums_f64(int n, const double* i1, const double* m1, const double* i2, const double* m2,
+                   double* out4) {
+  double s[4] = {0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    double d1 = (1.0 + std::fabs(i2[i] - m2[i])) / (1.0 + std::fabs(i1[i] - m1[i])) - 1.0;
+    double art = std::max(d1, 0.0), det = std::max(-d1, 0.0);
+    s[0] += art;
+    s[1] += (art * art) * (art * art);
+    s[2] += det;
+    s[3] += (det * det) * (det * det);
+  }
+  for (int k = 0; k < 4; ++k) out4[k] = s[k];
+}
+
+// ------------------------------------------------------------------ pipeline
+
+// cpu.rs:342-410 compute_frame_ssimulacra2. Inputs are packed linear RGB in
+// the working type T. Writes per-scale sums (18 per scale) and dims.
+template <typename T>
+int pipeline(const T* ref_lin, const T* dis_lin, int w0, int h0, int mode, double* sums,
+             int* dims) {
+  std::vector<T> img1(ref_lin, ref_lin + size_t(w0) * h0 * 3);
+  std::vector<T> img2(dis_lin, dis_lin + size_t(w0) * h0 * 3);
+  int w = w0, h = h0, nscales = 0;
+  std::vector<T> p1[3], p2[3], mul, s11, s22, s12, mu1[3], mu2[3], tmp, t1, t2;
+  for (int scale = 0; scale < NUM_SCALES; ++scale) {
+    if (w < 8 || h < 8) break;
+    if (scale > 0) {
+      int ow = (w + 1) / 2, oh = (h + 1) / 2;
+      t1.assign(size_t(ow) * oh * 3, T(0));
+      t2.assign(size_t(ow) * oh * 3, T(0));
+      downscale_packed(img1.data(), w, h, t1.data(), ow, oh);
+      downscale_packed(img2.data(), w, h, t2.data(), ow, oh);
+      img1.swap(t1);
+      img2.swap(t2);
+      w = ow;
+      h = oh;
+    }
+    const int n = w * h;
+    for (int c = 0; c < 3; ++c) {
+      p1[c].resize(n);
+      p2[c].resize(n);
+    }
+    for (int i = 0; i < n; ++i) {
+      T a[3], b[3];
+      if constexpr (sizeof(T) == 4) {
+        xyb_f32(img1[i * 3], img1[i * 3 + 1], img1[i * 3 + 2], &a[0], &a[1], &a[2]);
+        xyb_f32(img2[i * 3], img2[i * 3 + 1], img2[i * 3 + 2], &b[0], &b[1], &b[2]);
+      } else {
+        xyb_f64(img1[i * 3], img1[i * 3 + 1], img1[i * 3 + 2], &a[0], &a[1], &a[2]);
+        xyb_f64(img2[i * 3], img2[i * 3 + 1], img2[i * 3 + 2], &b[0], &b[1], &b[2]);
+      }
+      for (int c = 0; c < 3; ++c) {
+        p1[c][i] = a[c];
+        p2[c][i] = b[c];
+      }
+    }
+    auto blur = [&](const std::vector<T>& in, std::vector<T>& out) {
+      out.resize(n);
+      if constexpr (sizeof(T) == 4) {
+        if (mode == 0)
+          blur_iir_f32(in.data(), out.data(), w, h, tmp);
+        else
+          blur_fir<float>(in.data(), out.data(), w, h, tmp);
+      } else {
+        blur_fir<double>(in.data(), out.data(), w, h, tmp);
+      }
+    };
+    mul.resize(n);
+    double* ssc = sums + size_t(scale) * 18;
+    for (int c = 0; c < 3; ++c) {
+      for (int i = 0; i < n; ++i) mul[i] = p1[c][i] * p1[c][i];
+      blur(mul, s11);
+      for (int i = 0; i < n; ++i) mul[i] = p2[c][i] * p2[c][i];
+      blur(mul, s22);
+      for (int i = 0; i < n; ++i) mul[i] = p1[c][i] * p2[c][i];
+      blur(mul, s12);
+      blur(p1[c], mu1[c]);
+      blur(p2[c], mu2[c]);
+      double e[4];
+      if constexpr (sizeof(T) == 4) {
+        ssim_sums_f32(n, mu1[c].data(), mu2[c].data(), s11.data(), s22.data(), s12.data(),
+                      &ssc[c * 6 + 0], &ssc[c * 6 + 1]);
+        edge_sums_f32(n, p1[c].data(), mu1[c].data(), p2[c].data(), mu2[c].data(), e);
+      } else {
+        ssim_sums_f64(n, mu1[c].data(), mu2[c].data(), s11.data(), s22.data(), s12.data(),
+                      &ssc[c * 6 + 0], &ssc[c * 6 + 1]);
+        edge_sums_f64(n, p1[c].data(), mu1[c].data(), p2[c].data(), mu2[c].data(), e);
+      }
+      for (int k = 0; k < 4; ++k) ssc[c * 6 + 2 + k] = e[k];
+    }
+    dims[scale * 2] = w;
+    dims[scale * 2 + 1] = h;
+    ++nscales;
+  }
+  return nscales;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- per-stage entry points (for stage parity tests) ----
+
+void oracle_srgb_lut(float* out256) { std::memcpy(out256, kLut, sizeof(kLut)); }
+
+void oracle_weights(double* out108) { std::memcpy(out108, kWeight, sizeof(kWeight)); }
+
+void oracle_srgb_to_linear(const uint8_t* in, size_t n, float* out) {
+  for (size_t i = 0; i < n; ++i) out[i] = kLut[in[i]];
+}
+
+// Packed RGB -> packed XYB (positive-shifted), f32 mirror.
+void oracle_linear_to_xyb_f32(const float* rgb, size_t npix, float* xyb) {
+  for (size_t i = 0; i < npix; ++i)
+    xyb_f32(rgb[i * 3], rgb[i * 3 + 1], rgb[i * 3 + 2], &xyb[i * 3], &xyb[i * 3 + 1], &xyb[i * 3 + 2]);
+}
+
+void oracle_linear_to_xyb_f64(const double* rgb, size_t npix, double* xyb) {
+  for (size_t i = 0; i < npix; ++i)
+    xyb_f64(rgb[i * 3], rgb[i * 3 + 1], rgb[i * 3 + 2], &xyb[i * 3], &xyb[i * 3 + 1], &xyb[i * 3 + 2]);
+}
+
+void oracle_downscale_f32(const float* in, int w, int h, float* out) {
+  downscale_packed(in, w, h, out, (w + 1) / 2, (h + 1) / 2);
+}
+
+void oracle_blur_f32(const float* in, int w, int h, float* out) {
+  std::vector<float> tmp;
+  blur_iir_f32(in, out, w, h, tmp);
+}
+
+void oracle_blur_rows_f32(const float* in, int w, int h, float* out) {
+  for (int y = 0; y < h; ++y) iir_row_f32(in + size_t(y) * w, out + size_t(y) * w, w);
+}
+
+void oracle_blur_cols_f32(const float* in, int w, int h, float* out) { iir_cols_f32(in, out, w, h); }
+
+void oracle_blur_fir_f32(const float* in, int w, int h, float* out) {
+  std::vector<float> tmp;
+  blur_fir<float>(in, out, w, h, tmp);
+}
+
+void oracle_blur_f64(const double* in, int w, int h, double* out) {
+  std::vector<double> tmp;
+  blur_fir<double>(in, out, w, h, tmp);
+}
+
+// Per-pixel f32 error-map sums for one channel (cpu.rs:581-683); out6 in the
+// sums layout {ssim S1, S4, art S1, S4, det S1, S4}.
+void oracle_map_sums_f32(int n, const float* src, const float* dis, const float* mu1,
+                         const float* mu2, const float* s11, const float* s22, const float* s12,
+                         double* out6) {
+  ssim_sums_f32(n, mu1, mu2, s11, s22, s12, &out6[0], &out6[1]);
+  edge_sums_f32(n, src, mu1, dis, mu2, out6 + 2);
+}
+
+// Scale count and sizes per cpu.rs:358-368 (the first sub-8 scale is scored).
+int oracle_scale_dims(int w, int h, int* dims) {
+  int n = 0;
+  for (int s = 0; s < NUM_SCALES; ++s) {
+    if (w < 8 || h < 8) break;
+    if (s > 0) {
+      w = (w + 1) / 2;
+      h = (h + 1) / 2;
+    }
+    dims[2 * s] = w;
+    dims[2 * s + 1] = h;
+    ++n;
+  }
+  return n;
+}
+
+// cpu.rs:633-634, 676-679 (averages) and cpu.rs:840-868 (score).
+double oracle_score_from_sums(const double* sums, const int* dims, int nscales) {
+  double ssim = 0.0;
+  int i = 0;
+  for (int c = 0; c < 3; ++c)
+    for (int s = 0; s < nscales; ++s) {
+      const double opp = 1.0 / double(size_t(dims[2 * s]) * size_t(dims[2 * s + 1]));
+      const double* q = sums + size_t(s) * 18 + c * 6;
+      const double ssim_avg[2] = {opp * q[0], std::sqrt(std::sqrt(opp * q[1]))};
+      const double art[2] = {opp * q[2], std::sqrt(std::sqrt(opp * q[3]))};
+      const double det[2] = {opp * q[4], std::sqrt(std::sqrt(opp * q[5]))};
+      for (int nrm = 0; nrm < 2; ++nrm) {
+        ssim = std::fma(kWeight[i++], std::fabs(ssim_avg[nrm]), ssim);
+        ssim = std::fma(kWeight[i++], std::fabs(art[nrm]), ssim);
+        ssim = std::fma(kWeight[i++], std::fabs(det[nrm]), ssim);
+      }
+    }
+  ssim *= 0.9562382616834844;
+  ssim = std::fma(6.248496625763138e-5 * ssim * ssim, ssim,
+                  std::fma(2.326765642916932, ssim, -0.020884521182843837 * ssim * ssim));
+  if (ssim > 0.0)
+    ssim = std::fma(std::pow(ssim, 0.6276336467831387), -10.0, 100.0);
+  else
+    ssim = 100.0;
+  return ssim;
+}
+
+// Packed linear RGB f32 pair -> score. sums (108) / dims (12) may be null.
+int oracle_score_linear(const float* ref, const float* dis, int w, int h, int mode, double* score,
+                        double* sums_out, int* dims_out) {
+  if (w <= 0 || h <= 0) return -1;
+  double sums[108] = {0};
+  int dims[12] = {0};
+  int ns;
+  if (mode == 1) {
+    std::vector<double> a(ref, ref + size_t(w) * h * 3), b(dis, dis + size_t(w) * h * 3);
+    ns = pipeline<double>(a.data(), b.data(), w, h, mode, sums, dims);
+  } else {
+    ns = pipeline<float>(ref, dis, w, h, mode, sums, dims);
+  }
+  *score = oracle_score_from_sums(sums, dims, ns);
+  if (sums_out) std::memcpy(sums_out, sums, sizeof(sums));
+  if (dims_out) std::memcpy(dims_out, dims, sizeof(dims));
+  return ns;
+}
+
+// Packed sRGB u8 pair -> score (cpu.rs CpuImg::from_srgb then the pipeline).
+int oracle_score_srgb(const uint8_t* ref, const uint8_t* dis, int w, int h, int mode, double* score,
+                      double* sums_out, int* dims_out) {
+  const size_t n = size_t(w) * h * 3;
+  std::vector<float> a(n), b(n);
+  oracle_srgb_to_linear(ref, n, a.data());
+  oracle_srgb_to_linear(dis, n, b.data());
+  return oracle_score_linear(a.data(), b.data(), w, h, mode, score, sums_out, dims_out);
+}
+
+// CPU throughput baseline: score `npairs` independent pairs on `nthreads`
+// std::threads (mode as above). Pairs are packed sRGB u8.
+int oracle_score_srgb_many(const uint8_t* const* refs, const uint8_t* const* diss, const int* ws,
+                           const int* hs, int npairs, int mode, int nthreads, double* scores) {
+  if (nthreads < 1) nthreads = 1;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nthreads; ++t)
+    pool.emplace_back([=]() {
+      for (int i = t; i < npairs; i += nthreads)
+        oracle_score_srgb(refs[i], diss[i], ws[i], hs[i], mode, &scores[i], nullptr, nullptr);
+    });
+  for (auto& th : pool) th.join();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""Shared fixtures: the package path, the CPU oracle (test infrastructure) and
+the synthetic generators."""
+import ctypes
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG_DIR = ROOT / "turbo-metrics_amd"
+ORACLE_SO = ROOT / "oracle" / "libssim2_oracle.so"
+GOLDEN = Path(__file__).resolve().parent / "golden"
+if str(PKG_DIR) not in sys.path:
+    sys.path.insert(0, str(PKG_DIR))
+if str(ROOT / "oracle") not in sys.path:
+    sys.path.insert(0, str(ROOT / "oracle"))
+
+MIRROR, F64, F32_FIR = 0, 1, 2
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+class Oracle:
+    """ctypes view of oracle/libssim2_oracle.so (test infrastructure only)."""
+
+    def __init__(self, path=ORACLE_SO):
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(str(path))
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.oracle_score_srgb.argtypes = [vp, vp, i, i, i, ctypes.POINTER(d), vp, vp]
+        L.oracle_score_linear.argtypes = [vp, vp, i, i, i, ctypes.POINTER(d), vp, vp]
+        L.oracle_score_from_sums.argtypes = [vp, vp, i]
+        L.oracle_score_from_sums.restype = d
+        L.oracle_scale_dims.argtypes = [i, i, vp]
+        L.oracle_srgb_lut.argtypes = [vp]
+        L.oracle_weights.argtypes = [vp]
+        L.oracle_linear_to_xyb_f32.argtypes = [vp, ctypes.c_size_t, vp]
+        L.oracle_linear_to_xyb_f64.argtypes = [vp, ctypes.c_size_t, vp]
+        L.oracle_downscale_f32.argtypes = [vp, i, i, vp]
+        for n in ("oracle_blur_f32", "oracle_blur_rows_f32", "oracle_blur_cols_f32", "oracle_blur_fir_f32",
+                  "oracle_blur_f64"):
+            getattr(L, n).argtypes = [vp, i, i, vp]
+        L.oracle_map_sums_f32.argtypes = [i] + [vp] * 7 + [vp]
+        L.oracle_score_srgb_many.argtypes = [vp, vp, vp, vp, i, i, i, vp]
+        self.L = L
+
+    def score_srgb(self, ref, dis, mode=MIRROR, with_sums=False):
+        h, w = ref.shape[:2]
+        r = np.ascontiguousarray(ref, np.uint8)
+        q = np.ascontiguousarray(dis, np.uint8)
+        s = ctypes.c_double()
+        sums = np.zeros(108)
+        dims = np.zeros(12, np.int32)
+        n = self.L.oracle_score_srgb(r.ctypes.data, q.ctypes.data, w, h, mode, ctypes.byref(s),
+                                     sums.ctypes.data, dims.ctypes.data)
+        if with_sums:
+            return s.value, sums.reshape(6, 3, 6), dims.reshape(6, 2), n
+        return s.value
+
+    def score_linear(self, ref, dis, mode=MIRROR, with_sums=False):
+        h, w = ref.shape[:2]
+        r = np.ascontiguousarray(ref, np.float32)
+        q = np.ascontiguousarray(dis, np.float32)
+        s = ctypes.c_double()
+        sums = np.zeros(108)
+        dims = np.zeros(12, np.int32)
+        n = self.L.oracle_score_linear(r.ctypes.data, q.ctypes.data, w, h, mode, ctypes.byref(s),
+                                       sums.ctypes.data, dims.ctypes.data)
+        if with_sums:
+            return s.value, sums.reshape(6, 3, 6), dims.reshape(6, 2), n
+        return s.value
+
+    def lut(self):
+        out = np.zeros(256, np.float32)
+        self.L.oracle_srgb_lut(out.ctypes.data)
+        return out
+
+    def weights(self):
+        out = np.zeros(108, np.float64)
+        self.L.oracle_weights(out.ctypes.data)
+        return out
+
+    def plane_op(self, name, plane):
+        a = np.ascontiguousarray(plane)
+        h, w = a.shape
+        out = np.zeros_like(a)
+        getattr(self.L, name)(a.ctypes.data, w, h, out.ctypes.data)
+        return out
+
+    def xyb_f32(self, rgb):
+        a = np.ascontiguousarray(rgb, np.float32)
+        out = np.zeros_like(a)
+        self.L.oracle_linear_to_xyb_f32(a.ctypes.data, a.size // 3, out.ctypes.data)
+        return out
+
+    def xyb_f64(self, rgb):
+        a = np.ascontiguousarray(rgb, np.float64)
+        out = np.zeros_like(a)
+        self.L.oracle_linear_to_xyb_f64(a.ctypes.data, a.size // 3, out.ctypes.data)
+        return out
+
+    def downscale(self, rgb):
+        a = np.ascontiguousarray(rgb, np.float32)
+        h, w = a.shape[:2]
+        out = np.zeros(((h + 1) // 2, (w + 1) // 2, 3), np.float32)
+        self.L.oracle_downscale_f32(a.ctypes.data, w, h, out.ctypes.data)
+        return out
+
+    def scale_dims(self, w, h):
+        d = np.zeros(12, np.int32)
+        n = self.L.oracle_scale_dims(w, h, d.ctypes.data)
+        return [tuple(d[2 * i:2 * i + 2]) for i in range(n)]
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    return Oracle()
+
+
+@pytest.fixture(scope="session")
+def synth():
+    from ssimulacra2_hip import synth as s
+    return s

@@ -1,0 +1,154 @@
+"""GPU parity of the exact f32 path against the CPU oracle (cpu.rs mirror).
+
+Every f32 plane the GPU computes is meant to equal the oracle's bit for bit
+(same operations, same order, glibc-compatible cbrtf), so the per-scale sums
+may differ only by the order of f64 additions and the score by far less than
+the 0.01 bar. Cases follow the reference's own test shapes: the workload
+configurations, odd and minimum sizes (ssim2-gpu tests/parity_lock.rs:437-554),
+identical inputs (cpu.rs gives exactly 100), cached reference == direct.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SCORE_TOL = 1e-6     # |score_gpu - score_oracle|; the published bar is 0.01
+SUMS_RTOL = 1e-9     # f64 sums: summation order only
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def s2():
+    import ssimulacra2_hip
+    return ssimulacra2_hip
+
+
+CASES = [
+    # (name, kind, seed, w, h, step)
+    ("noise_grad_512", 1, 1, 512, 512, 0),
+    ("rect_dct_1080p", 2, 2, 1920, 1080, 16),
+    ("grad_dct_300x200", 4, 3, 300, 200, 8),
+    ("grad_dct_97x1500", 4, 4, 97, 1500, 40),
+    ("grad_dct_64", 4, 5, 64, 64, 20),
+    ("rect_dct_777x555", 2, 6, 777, 555, 4),
+    ("odd_123x45", 1, 7, 123, 45, 0),
+    ("min_8x8", 1, 8, 8, 8, 0),
+    ("tiny_16x16", 4, 9, 16, 16, 24),
+    ("thin_9x300", 1, 10, 9, 300, 0),
+    ("wide_1500x11", 2, 11, 1500, 11, 12),
+    ("odd_33x65", 4, 12, 33, 65, 30),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_score_matches_oracle(case, s2, oracle, synth):
+    name, kind, seed, w, h, step = case
+    ref, dis = synth.pair(kind, seed, w, h, step or 16.0)
+    cpu, cpu_sums, cpu_dims, n = oracle.score_srgb(ref, dis, with_sums=True)
+    _say(f"[gpu] {name} {w}x{h}")
+    with s2.Ssimulacra2(w, h) as m:
+        gpu = m.compute_from_cpu_srgb_sync(ref, dis)
+        sums, dims, ns = m.sums()
+    _say(f"[gpu] {name}: gpu {gpu:.9f} oracle {cpu:.9f} diff {gpu - cpu:+.3e}")
+    assert ns == n
+    np.testing.assert_array_equal(dims[:n], cpu_dims[:n])
+    np.testing.assert_allclose(sums[:n], cpu_sums[:n], rtol=SUMS_RTOL, atol=1e-300)
+    assert abs(gpu - cpu) <= SCORE_TOL
+
+
+def test_identical_scores_100(s2, synth):
+    ref, _ = synth.pair(2, 21, 640, 360, 16)
+    _say("[gpu] identical 640x360")
+    with s2.Ssimulacra2(640, 360) as m:
+        assert m.compute_from_cpu_srgb_sync(ref, ref) == 100.0
+
+
+def test_linear_and_srgb_device_paths_agree(s2, oracle, synth):
+    w, h = 200, 150
+    ref, dis = synth.pair(1, 31, w, h)
+    lut = oracle.lut()
+    ref_lin, dis_lin = lut[ref], lut[dis]
+    _say("[gpu] device paths 200x150")
+    with s2.Ssimulacra2(w, h) as m:
+        dr, dd = s2.DeviceImage.from_host(ref), s2.DeviceImage.from_host(dis)
+        lr = s2.DeviceImage(w, h, np.float32)
+        ld = s2.DeviceImage(w, h, np.float32)
+        a = m.compute_srgb_sync(dr, dd, lr, ld)
+        # the linear copies are the reference's srgb_to_linear output
+        np.testing.assert_array_equal(lr.download(), ref_lin)
+        np.testing.assert_array_equal(ld.download(), dis_lin)
+        b = m.compute_sync(ref_linear=s2.DeviceImage.from_host(ref_lin),
+                           dis_linear=s2.DeviceImage.from_host(dis_lin))
+        m.compute_srgb(dr, dd)
+        c = m.get_score()
+        assert m.get_score() == c  # idempotent
+    cpu = oracle.score_linear(ref_lin, dis_lin)
+    assert a == b == c
+    assert abs(a - cpu) <= SCORE_TOL
+
+
+def test_srgb_to_linear_kernel_is_the_table(s2, oracle):
+    w, h = 256, 3
+    img = np.zeros((h, w, 3), np.uint8)
+    img[..., 0] = np.arange(256, dtype=np.uint8)[None, :]
+    img[..., 1] = 255 - np.arange(256, dtype=np.uint8)[None, :]
+    img[..., 2] = (np.arange(256, dtype=np.uint8) * 7)[None, :]
+    _say("[gpu] srgb_to_linear")
+    src = s2.DeviceImage.from_host(img)
+    dst = s2.DeviceImage(w, h, np.float32)
+    s2.srgb_to_linear(src, dst)
+    np.testing.assert_array_equal(dst.download(), oracle.lut()[img])
+
+
+def test_cached_reference_equals_direct(s2, synth):
+    w, h = 320, 240
+    ref, d1 = synth.pair(4, 41, w, h, 8)
+    _, d2 = synth.pair(4, 42, w, h, 40)
+    _say("[gpu] cached reference 320x240")
+    with s2.Ssimulacra2(w, h) as m:
+        direct = [m.compute_from_cpu_srgb_sync(ref, d) for d in (d1, d2)]
+        assert not m.has_reference()
+        with pytest.raises(s2.Ssim2Error) as ei:
+            m.compute_with_reference_srgb(s2.DeviceImage.from_host(d1))
+        assert ei.value.status == "SSIM2_NO_REFERENCE"
+        m.set_reference_srgb(s2.DeviceImage.from_host(ref))
+        assert m.has_reference()
+        cached = [m.compute_with_reference_srgb(s2.DeviceImage.from_host(d)) for d in (d1, d2)]
+        # a full compute in between leaves the cache intact
+        again = m.compute_from_cpu_srgb_sync(ref, d2)
+        cached2 = m.compute_with_reference_srgb(s2.DeviceImage.from_host(d1))
+        m.clear_reference()
+        assert not m.has_reference()
+    assert cached == direct
+    assert again == direct[1] and cached2 == direct[0]
+
+
+def test_repeated_and_interleaved_sizes(s2, oracle, synth):
+    """Contexts of several sizes alive at once, reused (drop_stress.rs:31-128 intent)."""
+    sizes = [(96, 64), (257, 129), (64, 200)]
+    ctxs = [s2.Ssimulacra2(w, h) for w, h in sizes]
+    try:
+        for rep in range(2):
+            for (w, h), m in zip(sizes, ctxs):
+                ref, dis = synth.pair(1, 100 + w + rep, w, h)
+                _say(f"[gpu] interleaved {w}x{h} rep {rep}")
+                assert abs(m.compute_from_cpu_srgb_sync(ref, dis) - oracle.score_srgb(ref, dis)) <= SCORE_TOL
+    finally:
+        for m in ctxs:
+            m.close()
+
+
+def test_errors(s2):
+    with pytest.raises(s2.Ssim2Error) as ei:
+        s2.Ssimulacra2(7, 100)
+    assert ei.value.status == "SSIM2_INVALID_SIZE"
+    with s2.Ssimulacra2(64, 64) as m:
+        a = s2.DeviceImage(64, 64, np.uint8)
+        b = s2.DeviceImage(65, 64, np.uint8)
+        with pytest.raises(s2.Ssim2Error) as ei:
+            m.compute_srgb_sync(a, b)
+        assert ei.value.status == "SSIM2_DIM_MISMATCH"
+        assert m.mem_usage() > 0

@@ -1,0 +1,452 @@
+// C ABI of the ssimulacra2 HIP engine (include/ssim2hip.h): contexts, memory
+// planning, launch sequencing and error handling. Host code only; the kernels
+// live in ssim2_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ssim2hip.h"
+#include "ssim2_internal.h"
+#include "ssim2_plan.h"
+
+using namespace ssim2;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+ssim2_status fail(ssim2_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+ssim2_status hip_fail(hipError_t e, const char* what) {
+  return fail(SSIM2_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return hip_fail(_e, #expr);   \
+  } while (0)
+
+}  // namespace
+
+struct ssim2_ctx {
+  int w = 0, h = 0;
+  PairPlan plan;
+  float* arena = nullptr;
+  double* partials = nullptr;
+  double* result = nullptr;       // device, kResultDoubles
+  double* result_host = nullptr;  // pinned
+  PairDesc* d_desc = nullptr;
+  WorkItem* d_items = nullptr;
+  unsigned char* stage = nullptr;  // two packed u8 images for host uploads
+  size_t stage_pitch = 0;
+  size_t bytes = 0;
+  bool has_ref = false;
+  bool have_result = false;
+  hipStream_t last = nullptr;
+};
+
+namespace {
+
+void release(ssim2_ctx* c) {
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->partials) (void)hipFree(c->partials);
+  if (c->result) (void)hipFree(c->result);
+  if (c->result_host) (void)hipHostFree(c->result_host);
+  if (c->d_desc) (void)hipFree(c->d_desc);
+  if (c->d_items) (void)hipFree(c->d_items);
+  if (c->stage) (void)hipFree(c->stage);
+  c->arena = nullptr;
+  c->partials = c->result = c->result_host = nullptr;
+  c->d_desc = nullptr;
+  c->d_items = nullptr;
+  c->stage = nullptr;
+}
+
+ssim2_status check_image(const ssim2_ctx* c, const ssim2_image* im, size_t bps, const char* name) {
+  if (!im || !im->ptr) return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + " is null");
+  if ((int)im->width != c->w || (int)im->height != c->h)
+    return fail(SSIM2_DIM_MISMATCH, std::string(name) + " is " + std::to_string(im->width) + "x" +
+                                        std::to_string(im->height) + ", context is " +
+                                        std::to_string(c->w) + "x" + std::to_string(c->h));
+  if (im->pitch_bytes < size_t(im->width) * 3 * bps)
+    return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + " pitch smaller than a row");
+  if (bps == 4 && (im->pitch_bytes % 4 != 0 || (reinterpret_cast<uintptr_t>(im->ptr) & 3)))
+    return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + " f32 image not 4-byte aligned");
+  return SSIM2_OK;
+}
+
+InputArgs make_inputs(const ssim2_image* ref, const ssim2_image* dis, int kind) {
+  InputArgs in{};
+  in.ref = ref ? static_cast<const unsigned char*>(ref->ptr) : nullptr;
+  in.dis = dis ? static_cast<const unsigned char*>(dis->ptr) : nullptr;
+  in.ref_pitch = ref ? (long long)ref->pitch_bytes : 0;
+  in.dis_pitch = dis ? (long long)dis->pitch_bytes : 0;
+  in.kind = kind;
+  in.use = 1;
+  return in;
+}
+
+// Enqueue the pipeline for the context's single pair.
+ssim2_status enqueue(ssim2_ctx* c, int sides, const InputArgs& in, hipStream_t s) {
+  const PairPlan& p = c->plan;
+  HIP_TRY(launch_pyramid(c->d_desc, c->d_items + p.off_pyr, p.n_pyr, sides, in, s));
+  HIP_TRY(launch_rows(c->d_desc, c->d_items + p.off_rows, p.n_rows, sides, s));
+  HIP_TRY(launch_cols(c->d_desc, c->d_items + p.off_cols, p.n_cols, sides, s));
+  if (sides != kSidesRef) {
+    HIP_TRY(launch_finalize(c->d_desc, 1, s));
+    HIP_TRY(hipMemcpyAsync(c->result_host, c->result, kResultDoubles * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    c->have_result = true;
+  }
+  c->last = s;
+  return SSIM2_OK;
+}
+
+ssim2_status wait_result(ssim2_ctx* c) {
+  if (!c->have_result) return fail(SSIM2_INVALID_ARGUMENT, "no computation has been enqueued");
+  HIP_TRY(hipStreamSynchronize(c->last));
+  return SSIM2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssim2_version(void) { return 100; }
+
+const char* ssim2_status_string(ssim2_status st) {
+  switch (st) {
+    case SSIM2_OK: return "SSIM2_OK";
+    case SSIM2_INVALID_SIZE: return "SSIM2_INVALID_SIZE";
+    case SSIM2_DIM_MISMATCH: return "SSIM2_DIM_MISMATCH";
+    case SSIM2_NO_REFERENCE: return "SSIM2_NO_REFERENCE";
+    case SSIM2_OOM: return "SSIM2_OOM";
+    case SSIM2_BATCH_SIZE: return "SSIM2_BATCH_SIZE";
+    case SSIM2_ADMISSION_REFUSED: return "SSIM2_ADMISSION_REFUSED";
+    case SSIM2_HIP_ERROR: return "SSIM2_HIP_ERROR";
+    case SSIM2_INVALID_ARGUMENT: return "SSIM2_INVALID_ARGUMENT";
+    case SSIM2_NOT_READY: return "SSIM2_NOT_READY";
+  }
+  return "SSIM2_UNKNOWN";
+}
+
+const char* ssim2_last_error(void) { return g_last_error.c_str(); }
+
+ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* options,
+                          ssim2_ctx** out) {
+  if (!out) return fail(SSIM2_INVALID_ARGUMENT, "out is null");
+  *out = nullptr;
+  if (options && options->struct_size != sizeof(ssim2_options))
+    return fail(SSIM2_INVALID_ARGUMENT, "ssim2_options.struct_size mismatch");
+  if (!size_supported(width, height))
+    return fail(SSIM2_INVALID_SIZE, "image size " + std::to_string(width) + "x" +
+                                        std::to_string(height) + " outside [8, " +
+                                        std::to_string(kMaxDim) + "] or too many pixels");
+  ssim2_ctx* c = new (std::nothrow) ssim2_ctx();
+  if (!c) return fail(SSIM2_OOM, "host allocation failed");
+  c->w = (int)width;
+  c->h = (int)height;
+  plan_pair(c->w, c->h, 0, 0, &c->plan);
+  const PairPlan& p = c->plan;
+  const size_t arena_bytes = p.arena_elems * sizeof(float);
+  const size_t part_bytes = size_t(p.nparts) * kNumSums * sizeof(double);
+  const size_t items_bytes = p.items.size() * sizeof(WorkItem);
+  c->bytes = arena_bytes + part_bytes + kResultDoubles * sizeof(double) + sizeof(PairDesc) + items_bytes;
+
+  // Pre-flight budget, as the reference refuses before allocating (lib.rs:88-112).
+  size_t budget = options ? (size_t)options->memory_budget : 0;
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) budget = free_b;
+  }
+  if (budget && c->bytes > budget) {
+    delete c;
+    return fail(SSIM2_OOM, "estimated footprint " + std::to_string(c->bytes) +
+                               " B exceeds the budget of " + std::to_string(budget) + " B");
+  }
+  hipError_t e = hipSuccess;
+  if ((e = hipMalloc(&c->arena, arena_bytes)) != hipSuccess ||
+      (e = hipMalloc(&c->partials, std::max<size_t>(part_bytes, 8))) != hipSuccess ||
+      (e = hipMalloc(&c->result, kResultDoubles * sizeof(double))) != hipSuccess ||
+      (e = hipHostMalloc(&c->result_host, kResultDoubles * sizeof(double), hipHostMallocDefault)) !=
+          hipSuccess ||
+      (e = hipMalloc(&c->d_desc, sizeof(PairDesc))) != hipSuccess ||
+      (e = hipMalloc(&c->d_items, items_bytes)) != hipSuccess) {
+    release(c);
+    delete c;
+    (void)hipGetLastError();
+    return fail(SSIM2_OOM, std::string("device allocation failed: ") + hipGetErrorString(e));
+  }
+  PairDesc d = p.desc;
+  d.arena = c->arena;
+  d.partials = c->partials;
+  d.result = c->result;
+  if ((e = hipMemcpy(c->d_desc, &d, sizeof(PairDesc), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(c->d_items, p.items.data(), items_bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+    release(c);
+    delete c;
+    return hip_fail(e, "descriptor upload");
+  }
+  std::memset(c->result_host, 0, kResultDoubles * sizeof(double));
+  *out = c;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_destroy(ssim2_ctx* c) {
+  if (!c) return SSIM2_OK;
+  if (c->last || c->have_result) (void)hipStreamSynchronize(c->last);
+  (void)hipDeviceSynchronize();
+  release(c);
+  delete c;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_mem_usage(const ssim2_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  *bytes = c->bytes + (c->stage ? 2 * c->stage_pitch * size_t(c->h) : 0);
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_compute_linear_async(ssim2_ctx* c, const ssim2_image* ref,
+                                        const ssim2_image* dis, ssim2_stream stream) {
+  if (!c) return fail(SSIM2_INVALID_ARGUMENT, "ctx is null");
+  ssim2_status st;
+  if ((st = check_image(c, ref, 4, "ref_linear")) != SSIM2_OK) return st;
+  if ((st = check_image(c, dis, 4, "dis_linear")) != SSIM2_OK) return st;
+  return enqueue(c, kSidesBoth, make_inputs(ref, dis, kInputLinearF32), (hipStream_t)stream);
+}
+
+ssim2_status ssim2_get_score(ssim2_ctx* c, double* score) {
+  if (!c || !score) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st = wait_result(c);
+  if (st != SSIM2_OK) return st;
+  *score = c->result_host[0];
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_compute_linear_sync(ssim2_ctx* c, const ssim2_image* ref, const ssim2_image* dis,
+                                       ssim2_stream stream, double* score) {
+  if (!score) return fail(SSIM2_INVALID_ARGUMENT, "score is null");
+  ssim2_status st = ssim2_compute_linear_async(c, ref, dis, stream);
+  if (st != SSIM2_OK) return st;
+  return ssim2_get_score(c, score);
+}
+
+ssim2_status ssim2_compute_srgb_async(ssim2_ctx* c, const ssim2_image* ref, const ssim2_image* dis,
+                                      ssim2_stream stream) {
+  if (!c) return fail(SSIM2_INVALID_ARGUMENT, "ctx is null");
+  ssim2_status st;
+  if ((st = check_image(c, ref, 1, "ref_srgb")) != SSIM2_OK) return st;
+  if ((st = check_image(c, dis, 1, "dis_srgb")) != SSIM2_OK) return st;
+  return enqueue(c, kSidesBoth, make_inputs(ref, dis, kInputSrgbU8), (hipStream_t)stream);
+}
+
+ssim2_status ssim2_compute_srgb_sync(ssim2_ctx* c, const ssim2_image* ref, const ssim2_image* dis,
+                                     ssim2_image* ref_lin, ssim2_image* dis_lin, ssim2_stream stream,
+                                     double* score) {
+  if (!c || !score) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st;
+  if ((st = check_image(c, ref, 1, "ref_srgb")) != SSIM2_OK) return st;
+  if ((st = check_image(c, dis, 1, "dis_srgb")) != SSIM2_OK) return st;
+  if (ref_lin && (st = check_image(c, ref_lin, 4, "ref_linear")) != SSIM2_OK) return st;
+  if (dis_lin && (st = check_image(c, dis_lin, 4, "dis_linear")) != SSIM2_OK) return st;
+  InputArgs in = make_inputs(ref, dis, kInputSrgbU8);
+  if (ref_lin) {
+    in.lin_ref_out = static_cast<float*>(ref_lin->ptr);
+    in.lin_ref_pitch = (long long)ref_lin->pitch_bytes;
+  }
+  if (dis_lin) {
+    in.lin_dis_out = static_cast<float*>(dis_lin->ptr);
+    in.lin_dis_pitch = (long long)dis_lin->pitch_bytes;
+  }
+  if ((st = enqueue(c, kSidesBoth, in, (hipStream_t)stream)) != SSIM2_OK) return st;
+  return ssim2_get_score(c, score);
+}
+
+ssim2_status ssim2_compute_from_host_srgb(ssim2_ctx* c, const uint8_t* ref, const uint8_t* dis,
+                                          size_t row_bytes, ssim2_stream stream, double* score) {
+  if (!c || !ref || !dis || !score) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  const size_t rb = size_t(c->w) * 3;
+  if (row_bytes < rb) return fail(SSIM2_INVALID_ARGUMENT, "row_bytes smaller than 3*width");
+  if (!c->stage) {
+    c->stage_pitch = (rb + 255) / 256 * 256;
+    hipError_t e = hipMalloc(&c->stage, 2 * c->stage_pitch * size_t(c->h));
+    if (e != hipSuccess) {
+      c->stage = nullptr;
+      (void)hipGetLastError();
+      return fail(SSIM2_OOM, "staging allocation failed");
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* sr = c->stage;
+  unsigned char* sd = c->stage + c->stage_pitch * size_t(c->h);
+  HIP_TRY(hipMemcpy2DAsync(sr, c->stage_pitch, ref, row_bytes, rb, c->h, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpy2DAsync(sd, c->stage_pitch, dis, row_bytes, rb, c->h, hipMemcpyHostToDevice, s));
+  ssim2_image ri{sr, c->stage_pitch, (uint32_t)c->w, (uint32_t)c->h};
+  ssim2_image di{sd, c->stage_pitch, (uint32_t)c->w, (uint32_t)c->h};
+  ssim2_status st = enqueue(c, kSidesBoth, make_inputs(&ri, &di, kInputSrgbU8), s);
+  if (st != SSIM2_OK) return st;
+  return ssim2_get_score(c, score);
+}
+
+// ---- cached reference
+
+static ssim2_status set_reference(ssim2_ctx* c, const ssim2_image* ref, int kind, hipStream_t s) {
+  c->has_ref = false;
+  ssim2_status st = enqueue(c, kSidesRef, make_inputs(ref, nullptr, kind), s);
+  if (st != SSIM2_OK) return st;
+  HIP_TRY(hipStreamSynchronize(s));
+  c->has_ref = true;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_set_reference_linear(ssim2_ctx* c, const ssim2_image* ref, ssim2_stream stream) {
+  if (!c) return fail(SSIM2_INVALID_ARGUMENT, "ctx is null");
+  ssim2_status st = check_image(c, ref, 4, "ref_linear");
+  if (st != SSIM2_OK) return st;
+  return set_reference(c, ref, kInputLinearF32, (hipStream_t)stream);
+}
+
+ssim2_status ssim2_set_reference_srgb(ssim2_ctx* c, const ssim2_image* ref, ssim2_stream stream) {
+  if (!c) return fail(SSIM2_INVALID_ARGUMENT, "ctx is null");
+  ssim2_status st = check_image(c, ref, 1, "ref_srgb");
+  if (st != SSIM2_OK) return st;
+  return set_reference(c, ref, kInputSrgbU8, (hipStream_t)stream);
+}
+
+static ssim2_status with_reference(ssim2_ctx* c, const ssim2_image* dis, int kind, hipStream_t s,
+                                   double* score) {
+  if (!c->has_ref)
+    return fail(SSIM2_NO_REFERENCE, "set_reference must be called before compute_with_reference");
+  ssim2_status st = enqueue(c, kSidesDis, make_inputs(nullptr, dis, kind), s);
+  if (st != SSIM2_OK) return st;
+  return ssim2_get_score(c, score);
+}
+
+ssim2_status ssim2_compute_with_reference_linear(ssim2_ctx* c, const ssim2_image* ref,
+                                                 const ssim2_image* dis, ssim2_stream stream,
+                                                 double* score) {
+  if (!c || !score) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st;
+  if (ref && (st = check_image(c, ref, 4, "ref_linear")) != SSIM2_OK) return st;
+  if ((st = check_image(c, dis, 4, "dis_linear")) != SSIM2_OK) return st;
+  return with_reference(c, dis, kInputLinearF32, (hipStream_t)stream, score);
+}
+
+ssim2_status ssim2_compute_with_reference_srgb(ssim2_ctx* c, const ssim2_image* dis,
+                                               ssim2_stream stream, double* score) {
+  if (!c || !score) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st = check_image(c, dis, 1, "dis_srgb");
+  if (st != SSIM2_OK) return st;
+  return with_reference(c, dis, kInputSrgbU8, (hipStream_t)stream, score);
+}
+
+ssim2_status ssim2_clear_reference(ssim2_ctx* c) {
+  if (!c) return fail(SSIM2_INVALID_ARGUMENT, "ctx is null");
+  c->has_ref = false;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_has_reference(const ssim2_ctx* c, int* has) {
+  if (!c || !has) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  *has = c->has_ref ? 1 : 0;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_get_sums(ssim2_ctx* c, double* sums, int* dims, int* nscales) {
+  if (!c) return fail(SSIM2_INVALID_ARGUMENT, "ctx is null");
+  ssim2_status st = wait_result(c);
+  if (st != SSIM2_OK) return st;
+  if (sums) std::memcpy(sums, c->result_host + 1, kSumsPerPair * sizeof(double));
+  if (dims)
+    for (int i = 0; i < 2 * kMaxScales; ++i) dims[i] = (int)c->result_host[1 + kSumsPerPair + i];
+  if (nscales) *nscales = c->plan.desc.nscales;
+  return SSIM2_OK;
+}
+
+// ---- utilities
+
+ssim2_status ssim2_srgb_to_linear(const ssim2_image* src, ssim2_image* dst, ssim2_stream stream) {
+  if (!src || !dst || !src->ptr || !dst->ptr) return fail(SSIM2_INVALID_ARGUMENT, "null image");
+  if (src->width != dst->width || src->height != dst->height)
+    return fail(SSIM2_DIM_MISMATCH, "source and destination sizes differ");
+  if (src->pitch_bytes < size_t(src->width) * 3 || dst->pitch_bytes < size_t(dst->width) * 12 ||
+      dst->pitch_bytes % 4)
+    return fail(SSIM2_INVALID_ARGUMENT, "pitch too small or misaligned");
+  HIP_TRY(launch_srgb_to_linear(static_cast<const unsigned char*>(src->ptr), (long long)src->pitch_bytes,
+                                static_cast<float*>(dst->ptr), (long long)dst->pitch_bytes,
+                                (int)src->width, (int)src->height, (hipStream_t)stream));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_image_alloc(uint32_t width, uint32_t height, uint32_t bps, ssim2_image* out) {
+  if (!out || (bps != 1 && bps != 4) || width == 0 || height == 0)
+    return fail(SSIM2_INVALID_ARGUMENT, "bad image_alloc arguments");
+  const size_t row = size_t(width) * 3 * bps;
+  const size_t pitch = (row + 255) / 256 * 256;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, pitch * height);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SSIM2_OOM, std::string("image allocation failed: ") + hipGetErrorString(e));
+  }
+  out->ptr = p;
+  out->pitch_bytes = pitch;
+  out->width = width;
+  out->height = height;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_image_free(ssim2_image* im) {
+  if (!im) return SSIM2_OK;
+  if (im->ptr) HIP_TRY(hipFree(im->ptr));
+  im->ptr = nullptr;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_image_upload(ssim2_image* im, const void* host, size_t host_row_bytes) {
+  if (!im || !im->ptr || !host) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (host_row_bytes < size_t(im->width) * 3)
+    return fail(SSIM2_INVALID_ARGUMENT, "host_row_bytes smaller than a u8 row");
+  const size_t row = std::min(host_row_bytes, im->pitch_bytes);
+  HIP_TRY(hipMemcpy2D(im->ptr, im->pitch_bytes, host, host_row_bytes, row, im->height,
+                      hipMemcpyHostToDevice));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_image_download(const ssim2_image* im, void* host, size_t host_row_bytes) {
+  if (!im || !im->ptr || !host) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  const size_t row = std::min(host_row_bytes, im->pitch_bytes);
+  HIP_TRY(hipMemcpy2D(host, host_row_bytes, im->ptr, im->pitch_bytes, row, im->height,
+                      hipMemcpyDeviceToHost));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_stream_create(ssim2_stream* out) {
+  if (!out) return fail(SSIM2_INVALID_ARGUMENT, "out is null");
+  hipStream_t s;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *out = s;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_stream_destroy(ssim2_stream stream) {
+  if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_stream_synchronize(ssim2_stream stream) {
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return SSIM2_OK;
+}
+
+}  // extern "C"

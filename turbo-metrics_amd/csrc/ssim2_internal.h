@@ -1,0 +1,100 @@
+// Internal device-side descriptors and kernel launchers of the ssimulacra2
+// HIP engine. Not part of the public C ABI (include/ssim2hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ssim2 {
+
+constexpr int kMaxScales = 6;
+// Quantities blurred per channel: ref^2, dis^2, ref*dis, ref, dis.
+constexpr int kNumQ = 5;
+// Per (scale, channel) error-map sums: ssim S1, ssim S4, artifact S1,
+// artifact S4, detail-loss S1, detail-loss S4 (S4 = sum of d^4).
+constexpr int kNumSums = 6;
+constexpr int kSumsPerPair = kMaxScales * 3 * kNumSums;  // 108
+
+// Tile / block geometry (compile-time; the launch builders use the same values).
+constexpr int kPyrTile = 32;      // scale-0 tile edge of the pyramid kernel
+constexpr int kRowBlock = 16;     // rows per row-pass block
+constexpr int kRowChunk = 32;     // columns scanned per LDS chunk (row pass)
+constexpr int kColBlock = 16;     // columns per column-pass block
+constexpr int kColChunk = 32;     // rows scanned per LDS chunk (column pass)
+constexpr int kThreads = 256;
+
+enum InputKind : int {
+  kInputSrgbU8 = 0,     // packed RGB u8, pitched
+  kInputLinearF32 = 1,  // packed RGB f32 linear, pitched
+};
+
+// Which sides a launch processes.
+enum Sides : int {
+  kSidesBoth = 0,  // reference and distorted
+  kSidesRef = 1,   // reference only (cached-reference preparation)
+  kSidesDis = 2,   // distorted only, reference state read from the cache
+};
+
+struct ScaleDesc {
+  int w, h;          // scale dimensions
+  int pitch;         // plane row pitch in elements (multiple of 64)
+  int colblocks;     // ceil(w / kColBlock)
+  long long plane;   // elements per plane (pitch * h)
+  long long xyb;     // element offset of the 6 XYB planes (ref X,Y,B, dis X,Y,B)
+  long long hrow;    // element offset of the 15 row-pass planes (c*5 + q)
+  long long cache;   // element offset of the 9 cached-reference planes: ref XYB c, blurred
+                     // ref^2 (sigma11) c, blurred ref (mu1) c, for c = 0..2
+  int part;          // index of the first partial-sum record of this scale
+  int pad;
+};
+
+struct PairDesc {
+  const unsigned char* in_ref;  // device input, layout per in_kind
+  const unsigned char* in_dis;
+  long long in_ref_pitch;       // bytes
+  long long in_dis_pitch;
+  float* lin_ref_out;           // optional packed f32 linear copy of the scale-0 input
+  float* lin_dis_out;
+  long long lin_ref_pitch, lin_dis_pitch;
+  int in_kind;
+  int nscales;
+  int w, h;
+  float* arena;                 // all planes of this pair
+  double* partials;             // partial-sum records, kNumSums doubles each
+  double* result;               // [0] score, [1..108] sums (scale*18 + c*6 + j), [109..120] dims
+  ScaleDesc sc[kMaxScales];
+};
+
+// Per-launch input override for single-pair launches: the pyramid kernel reads
+// the images named here instead of the descriptor's, so a context's device
+// descriptor never changes between calls.
+struct InputArgs {
+  const unsigned char* ref;
+  const unsigned char* dis;
+  long long ref_pitch, dis_pitch;
+  float* lin_ref_out;
+  float* lin_dis_out;
+  long long lin_ref_pitch, lin_dis_pitch;
+  int kind;
+  int use;  // 0: take inputs from the PairDesc
+};
+
+// One work item per block: {pair, a, b, c}; the meaning depends on the kernel.
+struct WorkItem {
+  int pair, a, b, c;
+};
+
+// Launchers (kernels in ssim2_kernels.hip). `pairs` and `items` are device
+// pointers; `nitems` blocks are launched.
+hipError_t launch_pyramid(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
+                          const InputArgs& in, hipStream_t stream);
+hipError_t launch_rows(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
+                       hipStream_t stream);
+hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
+                       hipStream_t stream);
+hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream);
+hipError_t launch_srgb_to_linear(const unsigned char* in, long long in_pitch, float* out,
+                                 long long out_pitch, int w, int h, hipStream_t stream);
+
+}  // namespace ssim2

@@ -1,0 +1,489 @@
+// ssimulacra2 HIP kernels for gfx950 (MI355X): the exact f32 path.
+//
+// The per-pair work is four launches, each a flat list of blocks (WorkItem)
+// so one launch can cover every scale of every pair in a batch:
+//
+//   k_pyramid  one block per 32x32 scale-0 tile: input -> linear (LUT or f32)
+//              -> XYB planes of every scale. The linear pyramid never leaves
+//              LDS: a 32x32 tile at scale 0 owns exactly its 16x16 .. 1x1
+//              sub-tiles at scales 1..5, including the ceil(n/2) clamped edge.
+//   k_rows     one block per (scale, channel, 16-row band): the row-direction
+//              recursive Gaussian of cpu.rs (row form), 15 recurrences per row
+//              (5 quantities x 3 cosine terms), one lane per recurrence; inputs
+//              and products staged in LDS in 32-column chunks.
+//   k_cols     one block per (scale, channel, 16-column strip): the column
+//              pass (column form), then SSIM / edge-difference error maps and
+//              f64 L1/L4 partial sums, all in the same block.
+//   k_finalize one block per pair: fixed-order reduction of the partial sums
+//              and the 108-feature weighted score, in f64 on the device.
+//
+// Arithmetic follows crates/ssimulacra2-cuda/examples/cpu.rs exactly (see
+// ssim2_math.h), so every f32 plane equals the CPU's bit for bit and the score
+// differs only by the order of the f64 sums.
+#include "ssim2_internal.h"
+#include "ssim2_math.h"
+#include "ssim2_tables.h"
+
+namespace ssim2 {
+
+__constant__ float c_lut[256] = SSIM2_SRGB8_LUT_INIT;
+__constant__ double c_weight[108] = SSIM2_WEIGHTS_INIT;
+
+namespace {
+
+template <int SIDES>
+struct SideTraits;
+// Quantity index q: 0 ref^2, 1 dis^2, 2 ref*dis, 3 ref, 4 dis.
+template <>
+struct SideTraits<kSidesBoth> {
+  static constexpr int NQ = 5;
+  static constexpr int NI = 2;
+  __device__ static int q(int i) { return i; }
+};
+template <>
+struct SideTraits<kSidesRef> {
+  static constexpr int NQ = 2;
+  static constexpr int NI = 1;
+  __device__ static int q(int i) { return i == 0 ? 0 : 3; }
+};
+template <>
+struct SideTraits<kSidesDis> {
+  static constexpr int NQ = 3;
+  static constexpr int NI = 1;
+  __device__ static int q(int i) { return i == 0 ? 1 : (i == 1 ? 2 : 4); }
+};
+
+__device__ __forceinline__ float quantity(int q, float a, float b) {
+  switch (q) {
+    case 0: return a * a;
+    case 1: return b * b;
+    case 2: return a * b;
+    case 3: return a;
+    default: return b;
+  }
+}
+
+// XYB plane of (side image, channel) at scale s for the given launch sides.
+// Both: ref -> xyb[c], dis -> xyb[3+c]. Ref: ref -> cache[c]. Dis: dis -> xyb[3+c].
+__device__ __forceinline__ float* xyb_plane(const PairDesc& P, int s, int img, int c, int sides) {
+  const ScaleDesc& S = P.sc[s];
+  if (img == 0 && sides != kSidesBoth) return P.arena + S.cache + (long long)c * S.plane;
+  return P.arena + S.xyb + (long long)(img * 3 + c) * S.plane;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- pyramid
+
+template <int SIDES>
+__global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict__ pairs,
+                                                      const WorkItem* __restrict__ items,
+                                                      InputArgs in) {
+  constexpr int NI = SideTraits<SIDES>::NI;
+  constexpr int T0 = kPyrTile;
+  __shared__ float bufA[NI * 3][T0][T0 + 1];          // scales 0, 2, 4
+  __shared__ float bufB[NI * 3][T0 / 2][T0 / 2 + 1];  // scales 1, 3, 5
+  const WorkItem it = items[blockIdx.x];
+  const PairDesc& P = pairs[it.pair];
+  const int tid = threadIdx.x;
+  const int tx = it.a, ty = it.b;
+  const int w = P.w, h = P.h;
+  if (!in.use) {
+    in.ref = P.in_ref;
+    in.dis = P.in_dis;
+    in.ref_pitch = P.in_ref_pitch;
+    in.dis_pitch = P.in_dis_pitch;
+    in.lin_ref_out = P.lin_ref_out;
+    in.lin_dis_out = P.lin_dis_out;
+    in.lin_ref_pitch = P.lin_ref_pitch;
+    in.lin_dis_pitch = P.lin_dis_pitch;
+    in.kind = P.in_kind;
+  }
+
+  // Scale 0: input -> linear (LDS) -> XYB planes.
+  for (int p = tid; p < T0 * T0; p += kThreads) {
+    const int lx = p % T0, ly = p / T0;
+    const int gx = tx * T0 + lx, gy = ty * T0 + ly;
+    if (gx >= w || gy >= h) continue;
+    for (int im = 0; im < NI; ++im) {
+      const int img = SIDES == kSidesDis ? 1 : im;
+      const unsigned char* base = img == 0 ? in.ref : in.dis;
+      const long long pitch = img == 0 ? in.ref_pitch : in.dis_pitch;
+      float rgb[3];
+      if (in.kind == kInputSrgbU8) {
+        const unsigned char* px = base + gy * pitch + gx * 3;
+        rgb[0] = c_lut[px[0]];
+        rgb[1] = c_lut[px[1]];
+        rgb[2] = c_lut[px[2]];
+        float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
+        if (lo) {
+          const long long lp = img == 0 ? in.lin_ref_pitch : in.lin_dis_pitch;
+          float* q = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(lo) + gy * lp) + gx * 3;
+          q[0] = rgb[0];
+          q[1] = rgb[1];
+          q[2] = rgb[2];
+        }
+      } else {
+        const float* px = reinterpret_cast<const float*>(base + gy * pitch) + gx * 3;
+        rgb[0] = px[0];
+        rgb[1] = px[1];
+        rgb[2] = px[2];
+      }
+      float X, Y, B;
+      linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
+      const long long off = (long long)gy * P.sc[0].pitch + gx;
+      xyb_plane(P, 0, img, 0, SIDES)[off] = X;
+      xyb_plane(P, 0, img, 1, SIDES)[off] = Y;
+      xyb_plane(P, 0, img, 2, SIDES)[off] = B;
+      for (int c = 0; c < 3; ++c) bufA[im * 3 + c][ly][lx] = rgb[c];
+    }
+  }
+
+  // Scales 1..n-1: 2x2 mean of the previous scale (cpu.rs:545-579), then XYB.
+  for (int s = 1; s < P.nscales; ++s) {
+    __syncthreads();
+    const int T = T0 >> s;
+    const int pw = P.sc[s - 1].w, ph = P.sc[s - 1].h;
+    const int cw = P.sc[s].w, ch = P.sc[s].h;
+    const int ox0 = tx * T, oy0 = ty * T;
+    for (int p = tid; p < T * T; p += kThreads) {
+      const int lx = p % T, ly = p / T;
+      const int gx = ox0 + lx, gy = oy0 + ly;
+      if (gx >= cw || gy >= ch) continue;
+      int xs[2], ys[2];
+      for (int k = 0; k < 2; ++k) {
+        xs[k] = min(2 * gx + k, pw - 1) - 2 * ox0;
+        ys[k] = min(2 * gy + k, ph - 1) - 2 * oy0;
+      }
+      for (int im = 0; im < NI; ++im) {
+        const int img = SIDES == kSidesDis ? 1 : im;
+        float rgb[3];
+        for (int c = 0; c < 3; ++c) {
+          float sum = 0.0f;
+          if (s & 1) {
+            sum += bufA[im * 3 + c][ys[0]][xs[0]];
+            sum += bufA[im * 3 + c][ys[0]][xs[1]];
+            sum += bufA[im * 3 + c][ys[1]][xs[0]];
+            sum += bufA[im * 3 + c][ys[1]][xs[1]];
+          } else {
+            sum += bufB[im * 3 + c][ys[0]][xs[0]];
+            sum += bufB[im * 3 + c][ys[0]][xs[1]];
+            sum += bufB[im * 3 + c][ys[1]][xs[0]];
+            sum += bufB[im * 3 + c][ys[1]][xs[1]];
+          }
+          rgb[c] = sum * 0.25f;
+          if (s & 1)
+            bufB[im * 3 + c][ly][lx] = rgb[c];
+          else
+            bufA[im * 3 + c][ly][lx] = rgb[c];
+        }
+        float X, Y, B;
+        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
+        const long long off = (long long)gy * P.sc[s].pitch + gx;
+        xyb_plane(P, s, img, 0, SIDES)[off] = X;
+        xyb_plane(P, s, img, 1, SIDES)[off] = Y;
+        xyb_plane(P, s, img, 2, SIDES)[off] = B;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- row pass
+
+template <int SIDES>
+__global__ __launch_bounds__(kThreads) void k_rows(const PairDesc* __restrict__ pairs,
+                                                   const WorkItem* __restrict__ items) {
+  constexpr int NQ = SideTraits<SIDES>::NQ;
+  constexpr int RB = kRowBlock, CX = kRowChunk, HALO = 2 * kRadius;  // left n-6, right n+4
+  constexpr int TW = CX + HALO;                                       // staged columns
+  __shared__ float Qs[NQ][RB][TW + 1];
+  __shared__ float Os[NQ * 3][RB][CX + 1];
+  const WorkItem it = items[blockIdx.x];
+  const PairDesc& P = pairs[it.pair];
+  const int s = it.a, c = it.b, y0 = it.c * RB;
+  const ScaleDesc& S = P.sc[s];
+  const int w = S.w, h = S.h, pitch = S.pitch;
+  const float* __restrict__ ra = xyb_plane(P, s, 0, c, SIDES);
+  const float* __restrict__ da = xyb_plane(P, s, 1, c, SIDES);
+  float* hbase = P.arena + S.hrow + (long long)(c * kNumQ) * S.plane;
+  const int tid = threadIdx.x;
+
+  // Scan lane: row r, quantity slot qi, cosine term k.
+  const bool scan = tid < NQ * 3 * RB;
+  const int r = tid % RB, qk = tid / RB;
+  const int qi = qk / 3, k = qk % 3;
+  const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
+  const float mp = k == 0 ? kMulPrev1 : (k == 1 ? kMulPrev3 : kMulPrev5);
+  float prev = 0.0f, prev2 = 0.0f;
+
+  for (int x0 = -kRadius + 1; x0 < w; x0 += CX) {
+    // Phase A: stage columns [x0-6, x0+CX+4) of both inputs, form quantities.
+    for (int e = tid; e < RB * TW; e += kThreads) {
+      const int rr = e / TW, j = e % TW;
+      const int col = x0 - kRadius - 1 + j, row = y0 + rr;
+      float a = 0.0f, b = 0.0f;
+      if (row < h && col >= 0 && col < w) {
+        const long long off = (long long)row * pitch + col;
+        a = ra[off];
+        if (SIDES != kSidesRef) b = da[off];
+      }
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) Qs[i][rr][j] = quantity(SideTraits<SIDES>::q(i), a, b);
+    }
+    __syncthreads();
+    // Phase B: CX sequential steps of the row-form recurrence (cpu.rs:967-1022).
+    if (scan) {
+#pragma unroll 8
+      for (int i = 0; i < CX; ++i) {
+        const float sum = Qs[qi][r][i] + Qs[qi][r][i + HALO];
+        float o = sum * mi;
+        o = fmaf(-1.0f, prev2, o);
+        prev2 = prev;
+        o = fmaf(mp, prev, o);
+        prev = o;
+        Os[qk][r][i] = o;
+      }
+    }
+    __syncthreads();
+    // Phase C: out = (o1 + o3) + o5, coalesced stores of the row-pass planes.
+    for (int e = tid; e < NQ * RB * CX; e += kThreads) {
+      const int i = e % CX, rr = (e / CX) % RB, q2 = e / (CX * RB);
+      const int n = x0 + i, row = y0 + rr;
+      if (n >= 0 && n < w && row < h) {
+        const float v = (Os[q2 * 3][rr][i] + Os[q2 * 3 + 1][rr][i]) + Os[q2 * 3 + 2][rr][i];
+        hbase[(long long)SideTraits<SIDES>::q(q2) * S.plane + (long long)row * pitch + n] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- column pass + maps
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <int SIDES>
+__global__ __launch_bounds__(kThreads) void k_cols(const PairDesc* __restrict__ pairs,
+                                                   const WorkItem* __restrict__ items) {
+  constexpr int NQ = SideTraits<SIDES>::NQ;
+  constexpr int CB = kColBlock, RY = kColChunk, HALO = 2 * kRadius;
+  constexpr int TH = RY + HALO;
+  // Slot strides are 16 mod 32 words so the two 16-lane groups of a half-wave
+  // (different slots, same columns) hit disjoint LDS banks.
+  constexpr int QSTRIDE = TH * CB + 16;
+  constexpr int OSTRIDE = RY * CB + 16;
+  __shared__ float Qs[NQ * QSTRIDE];
+  __shared__ float Os[NQ * 3 * OSTRIDE];
+  __shared__ double red[kThreads / 64][kNumSums];
+  const WorkItem it = items[blockIdx.x];
+  const PairDesc& P = pairs[it.pair];
+  const int s = it.a, c = it.b, cb = it.c, xb = cb * CB;
+  const ScaleDesc& S = P.sc[s];
+  const int w = S.w, h = S.h, pitch = S.pitch;
+  const float* hbase = P.arena + S.hrow + (long long)(c * kNumQ) * S.plane;
+  const float* __restrict__ srcp = xyb_plane(P, s, 0, c, SIDES);
+  const float* __restrict__ disp = xyb_plane(P, s, 1, c, SIDES);
+  float* cache_s11 = P.arena + S.cache + (long long)(3 + c) * S.plane;
+  float* cache_mu1 = P.arena + S.cache + (long long)(6 + c) * S.plane;
+  const int tid = threadIdx.x;
+
+  const bool scan = tid < NQ * 3 * CB;
+  const int x = tid % CB, qk = tid / CB;
+  const int qi = qk / 3, k = qk % 3;
+  const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
+  const float vmp = k == 0 ? kVertMulPrev1 : (k == 1 ? kVertMulPrev3 : kVertMulPrev5);
+  float prev = 0.0f, prev2 = 0.0f;
+  double acc[kNumSums] = {0, 0, 0, 0, 0, 0};
+
+  for (int y0 = -kRadius + 1; y0 < h; y0 += RY) {
+    // Phase A: stage rows [y0-6, y0+RY+4) of the row-pass planes.
+    for (int e = tid; e < NQ * TH * CB; e += kThreads) {
+      const int q2 = e / (TH * CB), rem = e % (TH * CB);
+      const int i = rem / CB, xx = rem % CB;
+      const int row = y0 - kRadius - 1 + i, col = xb + xx;
+      float v = 0.0f;
+      if (row >= 0 && row < h && col < w)
+        v = hbase[(long long)SideTraits<SIDES>::q(q2) * S.plane + (long long)row * pitch + col];
+      Qs[q2 * QSTRIDE + i * CB + xx] = v;
+    }
+    __syncthreads();
+    // Phase B: column-form recurrence (cpu.rs:1054-1115).
+    if (scan) {
+      const float* q = Qs + qi * QSTRIDE + x;
+#pragma unroll 8
+      for (int i = 0; i < RY; ++i) {
+        const float sum = q[i * CB] + q[(i + HALO) * CB];
+        const float t = fmaf(prev, vmp, prev2);
+        const float o = fmaf(sum, mi, -t);
+        prev2 = prev;
+        prev = o;
+        Os[qk * OSTRIDE + i * CB + x] = o;
+      }
+    }
+    __syncthreads();
+    // Phase C: blurred quantities -> error maps -> f64 sums (or cache writes).
+    for (int e = tid; e < RY * CB; e += kThreads) {
+      const int i = e / CB, xx = e % CB;
+      const int n = y0 + i, col = xb + xx;
+      if (n < 0 || n >= h || col >= w) continue;
+      float bq[NQ];
+#pragma unroll
+      for (int q2 = 0; q2 < NQ; ++q2)
+        bq[q2] = (Os[(q2 * 3) * OSTRIDE + e] + Os[(q2 * 3 + 1) * OSTRIDE + e]) + Os[(q2 * 3 + 2) * OSTRIDE + e];
+      const long long off = (long long)n * pitch + col;
+      if (SIDES == kSidesRef) {
+        cache_s11[off] = bq[0];
+        cache_mu1[off] = bq[1];
+        continue;
+      }
+      float s11, s22, s12, mu1, mu2;
+      if (SIDES == kSidesBoth) {
+        s11 = bq[0];
+        s22 = bq[1];
+        s12 = bq[2];
+        mu1 = bq[3];
+        mu2 = bq[4];
+      } else {
+        s11 = cache_s11[off];
+        mu1 = cache_mu1[off];
+        s22 = bq[0];
+        s12 = bq[1];
+        mu2 = bq[2];
+      }
+      const double d = ssim_error(mu1, mu2, s11, s22, s12);
+      const double d2 = d * d;
+      acc[0] += d;
+      acc[1] += d2 * d2;
+      const double d1 = edge_ratio(srcp[off], mu1, disp[off], mu2);
+      const double art = d1 > 0.0 ? d1 : 0.0;
+      const double det = -d1 > 0.0 ? -d1 : 0.0;
+      const double a2 = art * art, e2 = det * det;
+      acc[2] += art;
+      acc[3] += a2 * a2;
+      acc[4] += det;
+      acc[5] += e2 * e2;
+    }
+  }
+  if (SIDES == kSidesRef) return;
+  // Block reduction in a fixed order: wave shuffles, then wave 0..3 in sequence.
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int j = 0; j < kNumSums; ++j) {
+    const double v = wave_sum(acc[j]);
+    if (lane == 0) red[wv][j] = v;
+  }
+  __syncthreads();
+  if (tid < kNumSums) {
+    double v = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
+    P.partials[(long long)(S.part + c * S.colblocks + cb) * kNumSums + tid] = v;
+  }
+}
+
+// ---------------------------------------------------------------- finalize
+
+// cpu.rs:633-634, 676-679 (averages) and cpu.rs:840-868 (weighted sum, transform).
+__global__ __launch_bounds__(128) void k_finalize(const PairDesc* __restrict__ pairs) {
+  const PairDesc& P = pairs[blockIdx.x];
+  __shared__ double sums[kSumsPerPair];
+  const int t = threadIdx.x;
+  if (t < kSumsPerPair) {
+    const int s = t / 18, c = (t / kNumSums) % 3, j = t % kNumSums;
+    double v = 0.0;
+    if (s < P.nscales) {
+      const ScaleDesc& S = P.sc[s];
+      const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums + j;
+      for (int b = 0; b < S.colblocks; ++b) v += p[(long long)b * kNumSums];
+    }
+    sums[t] = v;
+    P.result[1 + t] = v;
+  }
+  if (t < 2 * kMaxScales) {
+    const int s = t / 2;
+    P.result[1 + kSumsPerPair + t] = s < P.nscales ? (double)(t & 1 ? P.sc[s].h : P.sc[s].w) : 0.0;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  double ssim = 0.0;
+  int i = 0;
+  for (int c = 0; c < 3; ++c)
+    for (int s = 0; s < P.nscales; ++s) {
+      const double opp = 1.0 / (double)((long long)P.sc[s].w * P.sc[s].h);
+      const double* q = sums + s * 18 + c * kNumSums;
+      const double l1[3] = {opp * q[0], opp * q[2], opp * q[4]};
+      const double l4[3] = {sqrt(sqrt(opp * q[1])), sqrt(sqrt(opp * q[3])), sqrt(sqrt(opp * q[5]))};
+      for (int m = 0; m < 3; ++m) ssim = fma(c_weight[i++], fabs(l1[m]), ssim);
+      for (int m = 0; m < 3; ++m) ssim = fma(c_weight[i++], fabs(l4[m]), ssim);
+    }
+  ssim *= 0.9562382616834844;
+  ssim = fma(6.248496625763138e-5 * ssim * ssim, ssim,
+             fma(2.326765642916932, ssim, -0.020884521182843837 * ssim * ssim));
+  ssim = ssim > 0.0 ? fma(pow(ssim, 0.6276336467831387), -10.0, 100.0) : 100.0;
+  P.result[0] = ssim;
+}
+
+// ---------------------------------------------------------------- utilities
+
+// crates/ssimulacra2-cuda-kernel/src/srgb.rs:58-70 (Kernel::srgb_to_linear):
+// packed u8 RGB -> packed f32 linear RGB through the 256-entry table.
+__global__ __launch_bounds__(kThreads) void k_srgb_to_linear(const unsigned char* __restrict__ in,
+                                                             long long in_pitch, float* __restrict__ out,
+                                                             long long out_pitch, int w3, int h) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= w3 || y >= h) return;
+  float* o = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(out) + y * out_pitch);
+  o[x] = c_lut[in[y * in_pitch + x]];
+}
+
+// ---------------------------------------------------------------- launchers
+
+#define SSIM2_DISPATCH_SIDES(KERNEL, GRID, BLOCK, STREAM, ...)                         \
+  do {                                                                                 \
+    if (sides == kSidesBoth)                                                           \
+      hipLaunchKernelGGL(KERNEL<kSidesBoth>, GRID, BLOCK, 0, STREAM, __VA_ARGS__);     \
+    else if (sides == kSidesRef)                                                       \
+      hipLaunchKernelGGL(KERNEL<kSidesRef>, GRID, BLOCK, 0, STREAM, __VA_ARGS__);      \
+    else                                                                               \
+      hipLaunchKernelGGL(KERNEL<kSidesDis>, GRID, BLOCK, 0, STREAM, __VA_ARGS__);      \
+  } while (0)
+
+hipError_t launch_pyramid(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
+                          const InputArgs& in, hipStream_t stream) {
+  if (nitems <= 0) return hipSuccess;
+  SSIM2_DISPATCH_SIDES(k_pyramid, dim3(nitems), dim3(kThreads), stream, pairs, items, in);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
+                       hipStream_t stream) {
+  if (nitems <= 0) return hipSuccess;
+  SSIM2_DISPATCH_SIDES(k_rows, dim3(nitems), dim3(kThreads), stream, pairs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
+                       hipStream_t stream) {
+  if (nitems <= 0) return hipSuccess;
+  SSIM2_DISPATCH_SIDES(k_cols, dim3(nitems), dim3(kThreads), stream, pairs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3(npairs), dim3(128), 0, stream, pairs);
+  return hipGetLastError();
+}
+
+hipError_t launch_srgb_to_linear(const unsigned char* in, long long in_pitch, float* out,
+                                 long long out_pitch, int w, int h, hipStream_t stream) {
+  if (w <= 0 || h <= 0) return hipSuccess;
+  const int w3 = w * 3;
+  hipLaunchKernelGGL(k_srgb_to_linear, dim3((w3 + kThreads - 1) / kThreads, h), dim3(kThreads), 0,
+                     stream, in, in_pitch, out, out_pitch, w3, h);
+  return hipGetLastError();
+}
+
+}  // namespace ssim2

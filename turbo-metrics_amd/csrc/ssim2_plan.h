@@ -1,0 +1,95 @@
+// Host-side memory and work planning for one image pair (shared by the
+// single-pair context and the batch engine).
+#pragma once
+
+#include <vector>
+
+#include "ssim2_internal.h"
+
+namespace ssim2 {
+
+constexpr int kMaxDim = 32768;
+constexpr long long kMaxPixels = 1LL << 28;
+// Device result record per pair: score, 108 sums, 12 dims.
+constexpr int kResultDoubles = 1 + kSumsPerPair + 2 * kMaxScales;
+// Planes per scale in the arena: 6 XYB + 15 row-pass + 9 cached-reference.
+constexpr int kPlanesPerScale = 6 + 3 * kNumQ + 9;
+
+inline bool size_supported(long long w, long long h) {
+  return w >= 8 && h >= 8 && w <= kMaxDim && h <= kMaxDim && w * h <= kMaxPixels;
+}
+
+// Scale count and sizes of crates/ssimulacra2-cuda/examples/cpu.rs:358-368:
+// the size test runs before downscaling, so the first scale below 8 px is
+// still scored.
+inline int scale_dims(int w, int h, int* sw, int* sh) {
+  int n = 0;
+  for (int s = 0; s < kMaxScales; ++s) {
+    if (w < 8 || h < 8) break;
+    if (s > 0) {
+      w = (w + 1) / 2;
+      h = (h + 1) / 2;
+    }
+    sw[s] = w;
+    sh[s] = h;
+    ++n;
+  }
+  return n;
+}
+
+struct PairPlan {
+  PairDesc desc{};               // arena/partials/result pointers left null
+  long long arena_elems = 0;     // floats
+  int nparts = 0;                // partial-sum records
+  std::vector<WorkItem> items;   // pyramid items, then rows, then cols
+  int off_pyr = 0, n_pyr = 0, off_rows = 0, n_rows = 0, off_cols = 0, n_cols = 0;
+};
+
+// Lay out one pair's planes starting at element `arena_base` and its partial
+// records starting at `part_base`; `pair` is written into every work item.
+inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPlan* out, int pair = 0) {
+  PairPlan& p = *out;
+  p = PairPlan{};
+  PairDesc& d = p.desc;
+  d.w = w;
+  d.h = h;
+  int sw[kMaxScales], sh[kMaxScales];
+  d.nscales = scale_dims(w, h, sw, sh);
+  long long off = arena_base;
+  int part = part_base;
+  for (int s = 0; s < d.nscales; ++s) {
+    ScaleDesc& S = d.sc[s];
+    S.w = sw[s];
+    S.h = sh[s];
+    S.pitch = (sw[s] + 63) / 64 * 64;
+    S.colblocks = (sw[s] + kColBlock - 1) / kColBlock;
+    S.plane = (long long)S.pitch * S.h;
+    S.xyb = off;
+    S.hrow = off + 6 * S.plane;
+    S.cache = off + (6 + 3 * kNumQ) * S.plane;
+    off += kPlanesPerScale * S.plane;
+    S.part = part;
+    part += 3 * S.colblocks;
+  }
+  p.arena_elems = off - arena_base;
+  p.nparts = part - part_base;
+  // Pyramid: 32x32 scale-0 tiles.
+  p.off_pyr = 0;
+  for (int ty = 0; ty < (h + kPyrTile - 1) / kPyrTile; ++ty)
+    for (int tx = 0; tx < (w + kPyrTile - 1) / kPyrTile; ++tx) p.items.push_back({pair, tx, ty, 0});
+  p.n_pyr = (int)p.items.size();
+  // Row pass: (scale, channel, 16-row band), largest scale first.
+  p.off_rows = (int)p.items.size();
+  for (int s = 0; s < d.nscales; ++s)
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < (sh[s] + kRowBlock - 1) / kRowBlock; ++b) p.items.push_back({pair, s, c, b});
+  p.n_rows = (int)p.items.size() - p.off_rows;
+  // Column pass + maps: (scale, channel, 16-column strip).
+  p.off_cols = (int)p.items.size();
+  for (int s = 0; s < d.nscales; ++s)
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < d.sc[s].colblocks; ++b) p.items.push_back({pair, s, c, b});
+  p.n_cols = (int)p.items.size() - p.off_cols;
+}
+
+}  // namespace ssim2

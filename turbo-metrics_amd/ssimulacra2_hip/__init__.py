@@ -1,0 +1,360 @@
+"""ssimulacra2 on MI355X: Python host mirror of the reference's Rust API.
+
+`Ssimulacra2` mirrors `ssimulacra2_cuda::Ssimulacra2`
+(crates/ssimulacra2-cuda/src/lib.rs:27-602) method for method on top of the C
+ABI in include/ssim2hip.h (libssim2hip.so, hand-written HIP kernels for
+gfx950). Images are device buffers (`DeviceImage`, the counterpart of
+cudarse-npp `Image<S, C<3>>`); streams are `Stream` objects or None for the
+default stream.
+
+The shared library must be built in-tree (``__graft_entry__.build()``); there is
+no CPU fallback, and importing this package fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libssim2hip.so"
+
+__all__ = [
+    "Ssimulacra2",
+    "DeviceImage",
+    "Stream",
+    "Ssim2Error",
+    "lib",
+    "LIB_PATH",
+    "STATUS",
+]
+
+STATUS = {
+    0: "SSIM2_OK",
+    1: "SSIM2_INVALID_SIZE",
+    2: "SSIM2_DIM_MISMATCH",
+    3: "SSIM2_NO_REFERENCE",
+    4: "SSIM2_OOM",
+    5: "SSIM2_BATCH_SIZE",
+    6: "SSIM2_ADMISSION_REFUSED",
+    7: "SSIM2_HIP_ERROR",
+    8: "SSIM2_INVALID_ARGUMENT",
+    9: "SSIM2_NOT_READY",
+}
+
+
+class Ssim2Error(RuntimeError):
+    """A non-OK ssim2_status; `.status` holds the name, e.g. "SSIM2_DIM_MISMATCH"."""
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.status = STATUS.get(code, f"SSIM2_{code}")
+        super().__init__(f"{self.status}: {message}")
+
+
+class _Image(ctypes.Structure):
+    _fields_ = [
+        ("ptr", ctypes.c_void_p),
+        ("pitch_bytes", ctypes.c_size_t),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+    ]
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("memory_budget", ctypes.c_uint64),
+    ]
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP extension first (python -c "
+            "'import __graft_entry__ as g; g.build()')"
+        )
+    L = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.POINTER
+    vp, u32, u64, sz, dbl = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t, ctypes.c_double
+    img = P(_Image)
+    sig = {
+        "ssim2_create": (u32, u32, P(_Options), P(vp)),
+        "ssim2_destroy": (vp,),
+        "ssim2_mem_usage": (vp, P(sz)),
+        "ssim2_compute_linear_async": (vp, img, img, vp),
+        "ssim2_get_score": (vp, P(dbl)),
+        "ssim2_compute_linear_sync": (vp, img, img, vp, P(dbl)),
+        "ssim2_compute_srgb_sync": (vp, img, img, img, img, vp, P(dbl)),
+        "ssim2_compute_srgb_async": (vp, img, img, vp),
+        "ssim2_compute_from_host_srgb": (vp, vp, vp, sz, vp, P(dbl)),
+        "ssim2_set_reference_linear": (vp, img, vp),
+        "ssim2_set_reference_srgb": (vp, img, vp),
+        "ssim2_compute_with_reference_linear": (vp, img, img, vp, P(dbl)),
+        "ssim2_compute_with_reference_srgb": (vp, img, vp, P(dbl)),
+        "ssim2_clear_reference": (vp,),
+        "ssim2_has_reference": (vp, P(ctypes.c_int)),
+        "ssim2_get_sums": (vp, P(dbl), P(ctypes.c_int), P(ctypes.c_int)),
+        "ssim2_srgb_to_linear": (img, img, vp),
+        "ssim2_image_alloc": (u32, u32, u32, img),
+        "ssim2_image_free": (img,),
+        "ssim2_image_upload": (img, vp, sz),
+        "ssim2_image_download": (img, vp, sz),
+        "ssim2_stream_create": (P(vp),),
+        "ssim2_stream_destroy": (vp,),
+        "ssim2_stream_synchronize": (vp,),
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = list(args)
+        f.restype = ctypes.c_int
+    L.ssim2_status_string.argtypes = [ctypes.c_int]
+    L.ssim2_status_string.restype = ctypes.c_char_p
+    L.ssim2_last_error.argtypes = []
+    L.ssim2_last_error.restype = ctypes.c_char_p
+    L.ssim2_version.argtypes = []
+    L.ssim2_version.restype = ctypes.c_int
+    return L
+
+
+lib = _load()
+
+
+def _check(code: int):
+    if code != 0:
+        raise Ssim2Error(code, lib.ssim2_last_error().decode(errors="replace"))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return stream.handle
+
+
+class Stream:
+    """A non-blocking hipStream_t (cudarse-driver CuStream, stream.rs:46)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib.ssim2_stream_create(ctypes.byref(h)))
+        self.handle = h
+
+    def sync(self):
+        _check(lib.ssim2_stream_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib.ssim2_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceImage:
+    """Packed-RGB device image, u8 or f32 (cudarse-npp Image<S, C<3>>)."""
+
+    def __init__(self, width: int, height: int, dtype=np.uint8):
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.dtype(np.uint8), np.dtype(np.float32)):
+            raise ValueError("DeviceImage holds u8 or f32 samples")
+        self._img = _Image()
+        _check(lib.ssim2_image_alloc(width, height, self.dtype.itemsize, ctypes.byref(self._img)))
+
+    @property
+    def width(self):
+        return self._img.width
+
+    @property
+    def height(self):
+        return self._img.height
+
+    @property
+    def pitch_bytes(self):
+        return self._img.pitch_bytes
+
+    @property
+    def ptr(self):
+        return self._img.ptr
+
+    def as_struct(self):
+        return ctypes.byref(self._img)
+
+    @classmethod
+    def from_host(cls, array: np.ndarray) -> "DeviceImage":
+        """Upload an (h, w, 3) u8 or f32 array (copy_from_cpu, image/mod.rs:410-439)."""
+        a = np.ascontiguousarray(array)
+        if a.ndim != 3 or a.shape[2] != 3:
+            raise ValueError("expected an (h, w, 3) array")
+        im = cls(a.shape[1], a.shape[0], a.dtype)
+        im.upload(a)
+        return im
+
+    def upload(self, array: np.ndarray):
+        a = np.ascontiguousarray(array, dtype=self.dtype)
+        if a.shape != (self.height, self.width, 3):
+            raise ValueError(f"shape {a.shape} != {(self.height, self.width, 3)}")
+        _check(lib.ssim2_image_upload(ctypes.byref(self._img), a.ctypes.data, a.strides[0]))
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 3), dtype=self.dtype)
+        _check(lib.ssim2_image_download(ctypes.byref(self._img), out.ctypes.data, out.strides[0]))
+        return out
+
+    def free(self):
+        if self._img.ptr:
+            lib.ssim2_image_free(ctypes.byref(self._img))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def srgb_to_linear(src: DeviceImage, dst: DeviceImage, stream: Stream | None = None):
+    """Kernel::srgb_to_linear (crates/ssimulacra2-cuda/src/kernel.rs:36-57)."""
+    _check(lib.ssim2_srgb_to_linear(src.as_struct(), dst.as_struct(), _stream_ptr(stream)))
+
+
+class Ssimulacra2:
+    """ssimulacra2_cuda::Ssimulacra2 for MI355X (crates/ssimulacra2-cuda/src/lib.rs:27-602).
+
+    Construct with the frame size (`Ssimulacra2(width, height)`) or, as in the
+    reference, with the two linear images (`Ssimulacra2.new(ref_lin, dis_lin)`),
+    which are then the default inputs of `compute` / `compute_sync`.
+    """
+
+    def __init__(self, width: int, height: int, memory_budget: int = 0):
+        opts = _Options(ctypes.sizeof(_Options), 0, memory_budget)
+        h = ctypes.c_void_p()
+        _check(lib.ssim2_create(width, height, ctypes.byref(opts), ctypes.byref(h)))
+        self._ctx = h
+        self.width, self.height = width, height
+        self._bound = None
+
+    # -- construction / lifecycle (lib.rs:80-197, 1195-1202)
+    @classmethod
+    def new(cls, ref_linear: DeviceImage, dis_linear: DeviceImage, stream=None, memory_budget: int = 0):
+        if (ref_linear.width, ref_linear.height) != (dis_linear.width, dis_linear.height):
+            raise Ssim2Error(2, "reference and distorted sizes differ")
+        s = cls(ref_linear.width, ref_linear.height, memory_budget)
+        s._bound = (ref_linear, dis_linear)
+        return s
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            lib.ssim2_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def mem_usage(self) -> int:
+        v = ctypes.c_size_t()
+        _check(lib.ssim2_mem_usage(self._ctx, ctypes.byref(v)))
+        return v.value
+
+    # -- scoring (lib.rs:329-429)
+    def compute(self, stream=None, ref_linear=None, dis_linear=None):
+        ref_linear, dis_linear = self._inputs(ref_linear, dis_linear)
+        _check(lib.ssim2_compute_linear_async(self._ctx, ref_linear.as_struct(), dis_linear.as_struct(),
+                                              _stream_ptr(stream)))
+
+    def get_score(self) -> float:
+        v = ctypes.c_double()
+        _check(lib.ssim2_get_score(self._ctx, ctypes.byref(v)))
+        return v.value
+
+    def compute_sync(self, stream=None, ref_linear=None, dis_linear=None) -> float:
+        ref_linear, dis_linear = self._inputs(ref_linear, dis_linear)
+        v = ctypes.c_double()
+        _check(lib.ssim2_compute_linear_sync(self._ctx, ref_linear.as_struct(), dis_linear.as_struct(),
+                                             _stream_ptr(stream), ctypes.byref(v)))
+        return v.value
+
+    def compute_srgb_sync(self, src_ref: DeviceImage, src_dis: DeviceImage, ref_linear=None,
+                          dis_linear=None, stream=None) -> float:
+        v = ctypes.c_double()
+        _check(lib.ssim2_compute_srgb_sync(
+            self._ctx, src_ref.as_struct(), src_dis.as_struct(),
+            ref_linear.as_struct() if ref_linear is not None else None,
+            dis_linear.as_struct() if dis_linear is not None else None,
+            _stream_ptr(stream), ctypes.byref(v)))
+        return v.value
+
+    def compute_srgb(self, src_ref: DeviceImage, src_dis: DeviceImage, stream=None):
+        """Asynchronous u8 path; pair with get_score()."""
+        _check(lib.ssim2_compute_srgb_async(self._ctx, src_ref.as_struct(), src_dis.as_struct(),
+                                            _stream_ptr(stream)))
+
+    def compute_from_cpu_srgb_sync(self, ref_bytes: np.ndarray, dis_bytes: np.ndarray, stream=None) -> float:
+        r = np.ascontiguousarray(ref_bytes, dtype=np.uint8)
+        d = np.ascontiguousarray(dis_bytes, dtype=np.uint8)
+        row = self.width * 3
+        if r.size != row * self.height or d.size != row * self.height:
+            raise Ssim2Error(2, "host buffers do not match the context size")
+        v = ctypes.c_double()
+        _check(lib.ssim2_compute_from_host_srgb(self._ctx, r.ctypes.data, d.ctypes.data, row,
+                                                _stream_ptr(stream), ctypes.byref(v)))
+        return v.value
+
+    # -- cached reference (lib.rs:440-602)
+    def set_reference_linear(self, ref_linear: DeviceImage, stream=None):
+        _check(lib.ssim2_set_reference_linear(self._ctx, ref_linear.as_struct(), _stream_ptr(stream)))
+
+    def set_reference_srgb(self, ref_srgb: DeviceImage, stream=None):
+        _check(lib.ssim2_set_reference_srgb(self._ctx, ref_srgb.as_struct(), _stream_ptr(stream)))
+
+    def compute_with_reference_linear(self, ref_linear, dis_linear: DeviceImage, stream=None) -> float:
+        v = ctypes.c_double()
+        _check(lib.ssim2_compute_with_reference_linear(
+            self._ctx, ref_linear.as_struct() if ref_linear is not None else None,
+            dis_linear.as_struct(), _stream_ptr(stream), ctypes.byref(v)))
+        return v.value
+
+    def compute_with_reference_srgb(self, dis_srgb: DeviceImage, stream=None) -> float:
+        v = ctypes.c_double()
+        _check(lib.ssim2_compute_with_reference_srgb(self._ctx, dis_srgb.as_struct(), _stream_ptr(stream),
+                                                     ctypes.byref(v)))
+        return v.value
+
+    def clear_reference(self):
+        _check(lib.ssim2_clear_reference(self._ctx))
+
+    def has_reference(self) -> bool:
+        v = ctypes.c_int()
+        _check(lib.ssim2_has_reference(self._ctx, ctypes.byref(v)))
+        return bool(v.value)
+
+    # -- diagnostics
+    def sums(self):
+        """(sums[6,3,6], dims[6,2], nscales) behind the last score."""
+        s = (ctypes.c_double * 108)()
+        d = (ctypes.c_int * 12)()
+        n = ctypes.c_int()
+        _check(lib.ssim2_get_sums(self._ctx, s, d, ctypes.byref(n)))
+        return (np.array(s[:], dtype=np.float64).reshape(6, 3, 6),
+                np.array(d[:], dtype=np.int32).reshape(6, 2), n.value)
+
+    def _inputs(self, ref_linear, dis_linear):
+        if ref_linear is None or dis_linear is None:
+            if self._bound is None:
+                raise Ssim2Error(8, "no images bound: pass ref_linear/dis_linear or use Ssimulacra2.new")
+            return self._bound
+        return ref_linear, dis_linear
