@@ -151,7 +151,24 @@ ssim2_status ssim2_has_reference(const ssim2_ctx* ctx, int* has_reference);
  * the last computation. Any pointer may be NULL. */
 ssim2_status ssim2_get_sums(ssim2_ctx* ctx, double* sums108, int* dims12, int* nscales);
 
+/* Per-kernel timing of the sRGB u8 pipeline: runs it `reps` times with HIP
+ * events around each of its four kernels on `stream` and writes the mean
+ * milliseconds per launch to ms[4] = {pyramid, row pass, column pass + maps,
+ * finalize}. No counterpart in the reference (its profiling hooks are
+ * cuProfilerStart/Stop, crates/cudarse/cudarse-driver/src/lib.rs:56-62). */
+ssim2_status ssim2_time_kernels(ssim2_ctx* ctx, const ssim2_image* ref_srgb,
+                                const ssim2_image* dis_srgb, ssim2_stream stream, int reps,
+                                float* ms);
+
 /* ---- utilities ------------------------------------------------------------ */
+
+/* Raw device memory (cudarse-driver async alloc, mem.rs:27-35) and a plain
+ * streaming copy kernel of `bytes` (a multiple of 16), the known-size kernel
+ * that byte counters are calibrated against. */
+ssim2_status ssim2_device_alloc(size_t bytes, void** out);
+ssim2_status ssim2_device_free(void* ptr);
+ssim2_status ssim2_device_copy(void* dst, const void* src, size_t bytes, ssim2_stream stream);
+ssim2_status ssim2_device_synchronize(void);
 
 /* Kernel::srgb_to_linear (crates/ssimulacra2-cuda/src/kernel.rs:36-57):
  * packed u8 sRGB -> packed f32 linear through the reference's 256-entry table. */

@@ -373,6 +373,77 @@ ssim2_status ssim2_get_sums(ssim2_ctx* c, double* sums, int* dims, int* nscales)
   return SSIM2_OK;
 }
 
+ssim2_status ssim2_time_kernels(ssim2_ctx* c, const ssim2_image* ref, const ssim2_image* dis,
+                                ssim2_stream stream, int reps, float* ms) {
+  if (!c || !ms || reps <= 0) return fail(SSIM2_INVALID_ARGUMENT, "bad time_kernels arguments");
+  ssim2_status st;
+  if ((st = check_image(c, ref, 1, "ref_srgb")) != SSIM2_OK) return st;
+  if ((st = check_image(c, dis, 1, "dis_srgb")) != SSIM2_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const PairPlan& p = c->plan;
+  const InputArgs in = make_inputs(ref, dis, kInputSrgbU8);
+  hipEvent_t ev[5];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  double acc[4] = {0, 0, 0, 0};
+  ssim2_status out = SSIM2_OK;
+  for (int r = 0; r < reps && out == SSIM2_OK; ++r) {
+    hipError_t e = hipSuccess;
+    if ((e = hipEventRecord(ev[0], s)) != hipSuccess ||
+        (e = launch_pyramid(c->d_desc, c->d_items + p.off_pyr, p.n_pyr, kSidesBoth, in, s)) != hipSuccess ||
+        (e = hipEventRecord(ev[1], s)) != hipSuccess ||
+        (e = launch_rows(c->d_desc, c->d_items + p.off_rows, p.n_rows, kSidesBoth, s)) != hipSuccess ||
+        (e = hipEventRecord(ev[2], s)) != hipSuccess ||
+        (e = launch_cols(c->d_desc, c->d_items + p.off_cols, p.n_cols, kSidesBoth, s)) != hipSuccess ||
+        (e = hipEventRecord(ev[3], s)) != hipSuccess ||
+        (e = launch_finalize(c->d_desc, 1, s)) != hipSuccess ||
+        (e = hipEventRecord(ev[4], s)) != hipSuccess || (e = hipEventSynchronize(ev[4])) != hipSuccess) {
+      out = hip_fail(e, "time_kernels");
+      break;
+    }
+    for (int k = 0; k < 4; ++k) {
+      float t = 0.0f;
+      if ((e = hipEventElapsedTime(&t, ev[k], ev[k + 1])) != hipSuccess) {
+        out = hip_fail(e, "hipEventElapsedTime");
+        break;
+      }
+      acc[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (out != SSIM2_OK) return out;
+  for (int k = 0; k < 4; ++k) ms[k] = (float)(acc[k] / reps);
+  c->have_result = true;
+  c->last = s;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_device_copy(void* dst, const void* src, size_t bytes, ssim2_stream stream) {
+  if (!dst || !src || (bytes % 16)) return fail(SSIM2_INVALID_ARGUMENT, "copy needs 16-byte multiples");
+  HIP_TRY(launch_copy(src, dst, (long long)bytes, (hipStream_t)stream));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_device_alloc(size_t bytes, void** out) {
+  if (!out || !bytes) return fail(SSIM2_INVALID_ARGUMENT, "bad alloc arguments");
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return fail(SSIM2_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_device_free(void* p) {
+  if (p) HIP_TRY(hipFree(p));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return SSIM2_OK;
+}
+
 // ---- utilities
 
 ssim2_status ssim2_srgb_to_linear(const ssim2_image* src, ssim2_image* dst, ssim2_stream stream) {

@@ -94,6 +94,7 @@ hipError_t launch_rows(const PairDesc* pairs, const WorkItem* items, int nitems,
 hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
                        hipStream_t stream);
 hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream);
+hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream);
 hipError_t launch_srgb_to_linear(const unsigned char* in, long long in_pitch, float* out,
                                  long long out_pitch, int w, int h, hipStream_t stream);
 

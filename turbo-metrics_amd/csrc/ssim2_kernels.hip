@@ -24,6 +24,8 @@
 #include "ssim2_math.h"
 #include "ssim2_tables.h"
 
+#include <algorithm>
+
 namespace ssim2 {
 
 __constant__ float c_lut[256] = SSIM2_SRGB8_LUT_INIT;
@@ -438,6 +440,14 @@ __global__ __launch_bounds__(kThreads) void k_srgb_to_linear(const unsigned char
   o[x] = c_lut[in[y * in_pitch + x]];
 }
 
+// Plain streaming copy, 16 bytes per lane per iteration: the known-size
+// reference kernel that hardware byte counters are calibrated against.
+__global__ __launch_bounds__(kThreads) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                   long long n16) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- launchers
 
 #define SSIM2_DISPATCH_SIDES(KERNEL, GRID, BLOCK, STREAM, ...)                         \
@@ -474,6 +484,15 @@ hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems,
 hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream) {
   if (npairs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_finalize, dim3(npairs), dim3(128), 0, stream, pairs);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream) {
+  const long long n16 = bytes / 16;
+  if (n16 <= 0) return hipSuccess;
+  const long long blocks = std::min<long long>((n16 + kThreads - 1) / kThreads, 8192);
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(kThreads), 0, stream,
+                     static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
   return hipGetLastError();
 }
 

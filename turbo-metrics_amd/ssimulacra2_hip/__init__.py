@@ -106,6 +106,11 @@ def _load():
         "ssim2_stream_create": (P(vp),),
         "ssim2_stream_destroy": (vp,),
         "ssim2_stream_synchronize": (vp,),
+        "ssim2_time_kernels": (vp, img, img, vp, ctypes.c_int, P(ctypes.c_float)),
+        "ssim2_device_alloc": (sz, P(vp)),
+        "ssim2_device_free": (vp,),
+        "ssim2_device_copy": (vp, vp, sz, vp),
+        "ssim2_device_synchronize": (),
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -210,6 +215,35 @@ class DeviceImage:
     def free(self):
         if self._img.ptr:
             lib.ssim2_image_free(ctypes.byref(self._img))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    """hipDeviceSynchronize."""
+    _check(lib.ssim2_device_synchronize())
+
+
+class DeviceBuffer:
+    """Raw device allocation (bytes), with the calibration copy kernel."""
+
+    def __init__(self, nbytes: int):
+        p = ctypes.c_void_p()
+        _check(lib.ssim2_device_alloc(nbytes, ctypes.byref(p)))
+        self.ptr, self.nbytes = p, nbytes
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None, stream=None):
+        _check(lib.ssim2_device_copy(self.ptr, other.ptr, nbytes or min(self.nbytes, other.nbytes),
+                                     _stream_ptr(stream)))
+
+    def free(self):
+        if self.ptr:
+            lib.ssim2_device_free(self.ptr)
+            self.ptr = None
 
     def __del__(self):
         try:
@@ -343,6 +377,13 @@ class Ssimulacra2:
         return bool(v.value)
 
     # -- diagnostics
+    def time_kernels(self, src_ref: DeviceImage, src_dis: DeviceImage, reps: int = 10, stream=None):
+        """Mean ms per launch of {pyramid, rows, cols, finalize} (HIP events on `stream`)."""
+        ms = (ctypes.c_float * 4)()
+        _check(lib.ssim2_time_kernels(self._ctx, src_ref.as_struct(), src_dis.as_struct(),
+                                      _stream_ptr(stream), reps, ms))
+        return dict(zip(("pyramid", "rows", "cols", "finalize"), [float(v) for v in ms]))
+
     def sums(self):
         """(sums[6,3,6], dims[6,2], nscales) behind the last score."""
         s = (ctypes.c_double * 108)()
