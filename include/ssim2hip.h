@@ -143,6 +143,112 @@ ssim2_status ssim2_compute_with_reference_srgb(ssim2_ctx* ctx, const ssim2_image
 ssim2_status ssim2_clear_reference(ssim2_ctx* ctx);
 ssim2_status ssim2_has_reference(const ssim2_ctx* ctx, int* has_reference);
 
+/* ---- batched fast form ---------------------------------------------------- */
+/* No counterpart in the reference's ssimulacra2-cuda (its closest relative is
+ * the same-size Ssim2Batch of crates/ssim2-gpu/src/pipeline_batch.rs:120-612).
+ * Many sRGB pairs of unrelated sizes are scored by three persistent kernels
+ * captured once in a hipGraph. Arithmetic: exact 9-tap FIR blur (the closed
+ * form of the reference's recursive Gaussian), statistics in shifted
+ * coordinates, f32 maps, f64 sums; XYB of scales >= 1 stored in `storage`. Its
+ * scores follow the FP64 evaluation of the metric closely and
+ * are not bit-identical to the exact path. */
+
+typedef enum {
+  SSIM2_STORAGE_F32 = 0,
+  SSIM2_STORAGE_F16 = 1,
+  SSIM2_STORAGE_BF16 = 2
+} ssim2_storage;
+
+typedef struct {
+  uint32_t struct_size;  /* sizeof(ssim2_batch_options) */
+  uint32_t storage;      /* ssim2_storage */
+  uint32_t max_pairs;    /* pairs per batch */
+  uint32_t reserved0;
+  uint64_t max_pixels;   /* sum of width*height over a batch */
+  uint64_t memory_budget;/* refuse creation above this many device bytes (0 = free memory) */
+} ssim2_batch_options;
+
+/* One pair of packed sRGB u8 device images of equal size. */
+typedef struct {
+  ssim2_image ref;
+  ssim2_image dis;
+} ssim2_pair;
+
+typedef struct ssim2_batch ssim2_batch;
+
+ssim2_status ssim2_batch_create(const ssim2_batch_options* options, ssim2_batch** out);
+ssim2_status ssim2_batch_destroy(ssim2_batch* batch);
+ssim2_status ssim2_batch_mem_usage(const ssim2_batch* batch, size_t* bytes);
+/* Enqueue one batch on `stream` (waits first for this object's previous batch). */
+ssim2_status ssim2_batch_compute_async(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
+                                       ssim2_stream stream);
+/* Scores of the last batch, in submission order (waits for it). */
+ssim2_status ssim2_batch_get_scores(ssim2_batch* batch, double* scores, uint32_t capacity);
+ssim2_status ssim2_batch_compute_sync(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
+                                      ssim2_stream stream, double* scores);
+/* done = 1 once the last batch has finished (non-blocking). */
+ssim2_status ssim2_batch_query(ssim2_batch* batch, int* done);
+/* Per-scale sums of pair `index` of the last batch (layout of ssim2_get_sums). */
+ssim2_status ssim2_batch_get_sums(ssim2_batch* batch, uint32_t index, double* sums108, int* dims12);
+/* Output tile shape (0: 32x16, 1: 64x16, 2: 32x32, 3: 16x16) and grid size in
+ * multiples of the CU count; ssim2_batch_autotune times every combination on
+ * the given pairs and keeps the fastest. `report` (optional) receives a table. */
+ssim2_status ssim2_batch_set_tiling(ssim2_batch* batch, uint32_t tile_cfg, uint32_t grid_mult);
+ssim2_status ssim2_batch_get_tiling(const ssim2_batch* batch, uint32_t* tile_cfg, uint32_t* grid_mult);
+ssim2_status ssim2_batch_autotune(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
+                                  ssim2_stream stream, uint32_t reps, char* report, size_t report_size);
+
+/* ---- serving --------------------------------------------------------------- */
+/* No counterpart in the reference (whose callers score one frame pair at a
+ * time, crates/turbo-metrics/src/lib.rs:271-363). A scheduler thread owns
+ * `slots` batch engines on separate streams; requests submitted from any
+ * thread wait in FIFO order and are admitted into the next idle slot while
+ * their device footprint fits `hbm_budget` and the slot's capacity, so
+ * batches launched while others run overlap on the device. Every result
+ * carries host timestamps (seconds, steady clock) for latency accounting. A
+ * request's score does not depend on which batch it joined. */
+
+typedef struct {
+  uint32_t struct_size;           /* sizeof(ssim2_server_options) */
+  uint32_t storage;               /* ssim2_storage of the batch engines */
+  uint32_t slots;                 /* batches in flight at most, 1..16 */
+  uint32_t max_pairs_per_batch;
+  uint64_t max_pixels_per_batch;  /* also the largest request accepted */
+  uint64_t hbm_budget;            /* admission limit on in-flight device bytes */
+  uint32_t tile_cfg;              /* as ssim2_batch_set_tiling; 0/0 = default */
+  uint32_t grid_mult;
+} ssim2_server_options;
+
+typedef struct {
+  uint64_t request_id;
+  double score;
+  double submit_time_s;
+  double done_time_s;
+  ssim2_status status;
+  uint32_t reserved0;
+} ssim2_result;
+
+typedef struct {
+  uint32_t struct_size;
+  uint32_t reserved0;
+  uint64_t submitted, completed, refused, batches, pixels_completed, waiting, in_flight_bytes;
+  double first_submit_s, last_done_s;
+} ssim2_server_stats_t;
+
+typedef struct ssim2_server ssim2_server;
+
+ssim2_status ssim2_server_create(const ssim2_server_options* options, ssim2_server** out);
+/* Stops the scheduler after the batches in flight and frees everything. */
+ssim2_status ssim2_server_destroy(ssim2_server* server);
+/* Thread-safe. SSIM2_ADMISSION_REFUSED when the request can never fit the
+ * server's limits; the images must stay valid until its result is polled. */
+ssim2_status ssim2_server_submit(ssim2_server* server, const ssim2_pair* pair, uint64_t request_id);
+/* Move up to `capacity` finished results out; *n receives the count. */
+ssim2_status ssim2_server_poll(ssim2_server* server, ssim2_result* out, uint32_t capacity, uint32_t* n);
+/* Wait until every accepted request has finished (timeout_s <= 0: no limit). */
+ssim2_status ssim2_server_drain(ssim2_server* server, double timeout_s);
+ssim2_status ssim2_server_stats(ssim2_server* server, ssim2_server_stats_t* stats);
+
 /* ---- diagnostics --------------------------------------------------------- */
 
 /* The raw per-scale sums behind the last score: sums[scale*18 + c*6 + j],

@@ -17,6 +17,23 @@
 //   MODE_F32_FIR    (2)  f32 everywhere like the mirror, but the blur is the
 //                        9-tap FIR in f32 (diagnostic: the spread between
 //                        blur evaluation orders).
+//   MODE_FAST       (3)  the arithmetic of the fast (batched / serving) GPU
+//                        form: correctly rounded f32 cube root, 9-tap FIR in
+//                        f32, error maps in f32, f64 sums.
+//   MODE_FAST_F16   (4)  MODE_FAST with the XYB planes of scales >= 1 stored
+//                        as IEEE binary16 of (value - channel shift), as the
+//                        FP16-storage form keeps them in HBM.
+//   MODE_FAST_F16_ALL (5) as 4, for every scale including scale 0.
+//   MODE_FAST_LIN16 (6)  diagnostic: linear pyramid of scales >= 1 in binary16.
+//   MODE_FAST_SHIFT (7)  MODE_FAST, but the five quantities are blurred in
+//                        shifted coordinates x' = x - shift(channel) and the
+//                        statistics are rebuilt with the exact zero-padding
+//                        corrections (b1 = blur of ones, e = 1 - b1):
+//                          var = (blur(x'^2) - blur(x')^2) + e*c*(2*blur(x') + c*b1)
+//                          cov = (blur(r'd') - blur(r')blur(d')) + e*c*(blur(r') + blur(d') + c*b1)
+//                          mu  = blur(x') + c*b1,   x - mu = (x' - blur(x')) + c*e
+//   MODE_FAST_SHIFT_F16 (8) MODE_FAST_SHIFT with the XYB planes of scales >= 1
+//                        stored in binary16 as in MODE_FAST_F16.
 //
 // Parity pins (each checked in tests/):
 //   * LUT and WEIGHT tables: equal, bit for bit, to the tables extracted from
@@ -222,6 +239,122 @@ void edge_sums_f32(int n, const float* i1, const float* m1, const float* i2, con
   for (int k = 0; k < 4; ++k) out4[k] = s[k];
 }
 
+// ------------------------------------------------------------------ fast form
+
+// IEEE binary16 round trip (round to nearest even) for finite |f| < 65520.
+inline float round_f16(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = x & 0x80000000u;
+  uint32_t ax = x & 0x7fffffffu;
+  if (ax < 0x38800000u) {  // below 2^-14: binary16 subnormal, quantum 2^-24
+    float a;
+    std::memcpy(&a, &ax, 4);
+    const float q = 0x1.0p-24f;
+    const float r = std::nearbyint(a / q) * q;
+    return sign ? -r : r;
+  }
+  const uint32_t lsb = (ax >> 13) & 1u;
+  ax += 0xfffu + lsb;
+  ax &= ~0x1fffu;
+  const uint32_t y = sign | ax;
+  float r;
+  std::memcpy(&r, &y, 4);
+  return r;
+}
+
+// Per-channel centre of the sRGB gamut in positive XYB (X 0.205..0.813,
+// Y 0.010..0.855, B 0.272..0.938): binary16 stores value - shift, so the
+// absolute rounding error is smallest where values cluster.
+const float kXybShift[3] = {0.5f, 0.4375f, 0.625f};
+
+inline void xyb_fast(float r, float g, float b, float* X, float* Y, float* B) {
+  float rg = std::fma(K_M00, r, std::fma(K_M01, g, std::fma(K_M02, b, K_B0)));
+  float gr = std::fma(K_M10, r, std::fma(K_M11, g, std::fma(K_M12, b, K_B0)));
+  float bb = std::fma(K_M20, r, std::fma(K_M21, g, std::fma(K_M22, b, K_B0)));
+  rg = (float)std::cbrt((double)std::max(rg, 0.0f)) - K_B0_ROOT;
+  gr = (float)std::cbrt((double)std::max(gr, 0.0f)) - K_B0_ROOT;
+  bb = (float)std::cbrt((double)std::max(bb, 0.0f)) - K_B0_ROOT;
+  float x = 0.5f * (rg - gr);
+  float y = 0.5f * (rg + gr);
+  *X = std::fma(x, 14.0f, 0.42f);
+  *Y = y + 0.01f;
+  *B = (bb - y) + 0.55f;
+}
+
+void map_sums_fast(int n, const float* src, const float* dis, const float* m1, const float* m2,
+                   const float* s11, const float* s22, const float* s12, double* out6) {
+  const float C2 = 0.0009f;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    float md = m1[i] - m2[i];
+    float num_m = std::fma(md, -md, 1.0f);
+    float num_s = std::fma(2.0f, s12[i] - m1[i] * m2[i], C2);
+    float den = ((s11[i] - m1[i] * m1[i]) + (s22[i] - m2[i] * m2[i])) + C2;
+    float d = std::max(1.0f - (num_m * num_s) / den, 0.0f);
+    float d2 = d * d;
+    a[0] += d;
+    a[1] += d2 * d2;
+    float d1 = (1.0f + std::fabs(dis[i] - m2[i])) / (1.0f + std::fabs(src[i] - m1[i])) - 1.0f;
+    float art = std::max(d1, 0.0f), det = std::max(-d1, 0.0f);
+    float a2 = art * art, e2 = det * det;
+    a[2] += art;
+    a[3] += a2 * a2;
+    a[4] += det;
+    a[5] += e2 * e2;
+  }
+  for (int k = 0; k < 6; ++k) out6[k] = a[k];
+}
+
+// MODE_FAST_SHIFT statistics for one scale (see the header for the identities).
+void shifted_scale_sums(const std::vector<float>* p1, const std::vector<float>* p2, int w, int h,
+                        double* ssc) {
+  const int n = w * h;
+  std::vector<float> ones(n, 1.0f), b1(n), a(n), b(n), q(n), m1(n), m2(n), s11(n), s22(n), s12(n), tmp;
+  blur_fir<float>(ones.data(), b1.data(), w, h, tmp);
+  const float C2 = 0.0009f;
+  for (int c = 0; c < 3; ++c) {
+    const float sh = kXybShift[c];
+    for (int i = 0; i < n; ++i) {
+      a[i] = p1[c][i] - sh;
+      b[i] = p2[c][i] - sh;
+    }
+    blur_fir<float>(a.data(), m1.data(), w, h, tmp);
+    blur_fir<float>(b.data(), m2.data(), w, h, tmp);
+    for (int i = 0; i < n; ++i) q[i] = a[i] * a[i];
+    blur_fir<float>(q.data(), s11.data(), w, h, tmp);
+    for (int i = 0; i < n; ++i) q[i] = b[i] * b[i];
+    blur_fir<float>(q.data(), s22.data(), w, h, tmp);
+    for (int i = 0; i < n; ++i) q[i] = a[i] * b[i];
+    blur_fir<float>(q.data(), s12.data(), w, h, tmp);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+      const float e = 1.0f - b1[i];
+      const float cb = sh * b1[i];
+      const float var1 = (s11[i] - m1[i] * m1[i]) + e * sh * (2.0f * m1[i] + cb);
+      const float var2 = (s22[i] - m2[i] * m2[i]) + e * sh * (2.0f * m2[i] + cb);
+      const float cov = (s12[i] - m1[i] * m2[i]) + e * sh * ((m1[i] + m2[i]) + cb);
+      const float md = m1[i] - m2[i];
+      const float num_m = std::fma(md, -md, 1.0f);
+      const float num_s = std::fma(2.0f, cov, C2);
+      const float den = (var1 + var2) + C2;
+      const float d = std::max(1.0f - (num_m * num_s) / den, 0.0f);
+      const float d2 = d * d;
+      acc[0] += d;
+      acc[1] += d2 * d2;
+      const float ce = sh * e;
+      const float d1 = (1.0f + std::fabs((b[i] - m2[i]) + ce)) / (1.0f + std::fabs((a[i] - m1[i]) + ce)) - 1.0f;
+      const float art = std::max(d1, 0.0f), det = std::max(-d1, 0.0f);
+      const float a2 = art * art, e2 = det * det;
+      acc[2] += art;
+      acc[3] += a2 * a2;
+      acc[4] += det;
+      acc[5] += e2 * e2;
+    }
+    for (int k = 0; k < 6; ++k) ssc[c * 6 + k] = acc[k];
+  }
+}
+
 // ------------------------------------------------------------------ f64 oracle
 
 inline void xyb_f64(double r, double g, double b, double* X, double* Y, double* B) {
@@ -305,8 +438,28 @@ int pipeline(const T* ref_lin, const T* dis_lin, int w0, int h0, int mode, doubl
     for (int i = 0; i < n; ++i) {
       T a[3], b[3];
       if constexpr (sizeof(T) == 4) {
-        xyb_f32(img1[i * 3], img1[i * 3 + 1], img1[i * 3 + 2], &a[0], &a[1], &a[2]);
-        xyb_f32(img2[i * 3], img2[i * 3 + 1], img2[i * 3 + 2], &b[0], &b[1], &b[2]);
+        if (mode >= 3) {
+          float l1[3], l2[3];
+          for (int c = 0; c < 3; ++c) {
+            l1[c] = img1[i * 3 + c];
+            l2[c] = img2[i * 3 + c];
+            if (mode == 6 && scale > 0) {
+              l1[c] = round_f16(l1[c]);
+              l2[c] = round_f16(l2[c]);
+            }
+          }
+          xyb_fast(l1[0], l1[1], l1[2], &a[0], &a[1], &a[2]);
+          xyb_fast(l2[0], l2[1], l2[2], &b[0], &b[1], &b[2]);
+          if (((mode == 4 || mode == 8) && scale > 0) || mode == 5) {
+            for (int c = 0; c < 3; ++c) {
+              a[c] = round_f16(a[c] - kXybShift[c]) + kXybShift[c];
+              b[c] = round_f16(b[c] - kXybShift[c]) + kXybShift[c];
+            }
+          }
+        } else {
+          xyb_f32(img1[i * 3], img1[i * 3 + 1], img1[i * 3 + 2], &a[0], &a[1], &a[2]);
+          xyb_f32(img2[i * 3], img2[i * 3 + 1], img2[i * 3 + 2], &b[0], &b[1], &b[2]);
+        }
       } else {
         xyb_f64(img1[i * 3], img1[i * 3 + 1], img1[i * 3 + 2], &a[0], &a[1], &a[2]);
         xyb_f64(img2[i * 3], img2[i * 3 + 1], img2[i * 3 + 2], &b[0], &b[1], &b[2]);
@@ -327,8 +480,17 @@ int pipeline(const T* ref_lin, const T* dis_lin, int w0, int h0, int mode, doubl
         blur_fir<double>(in.data(), out.data(), w, h, tmp);
       }
     };
-    mul.resize(n);
     double* ssc = sums + size_t(scale) * 18;
+    if constexpr (sizeof(T) == 4) {
+      if (mode == 7 || mode == 8) {
+        shifted_scale_sums(p1, p2, w, h, ssc);
+        dims[scale * 2] = w;
+        dims[scale * 2 + 1] = h;
+        ++nscales;
+        continue;
+      }
+    }
+    mul.resize(n);
     for (int c = 0; c < 3; ++c) {
       for (int i = 0; i < n; ++i) mul[i] = p1[c][i] * p1[c][i];
       blur(mul, s11);
@@ -340,9 +502,18 @@ int pipeline(const T* ref_lin, const T* dis_lin, int w0, int h0, int mode, doubl
       blur(p2[c], mu2[c]);
       double e[4];
       if constexpr (sizeof(T) == 4) {
-        ssim_sums_f32(n, mu1[c].data(), mu2[c].data(), s11.data(), s22.data(), s12.data(),
-                      &ssc[c * 6 + 0], &ssc[c * 6 + 1]);
-        edge_sums_f32(n, p1[c].data(), mu1[c].data(), p2[c].data(), mu2[c].data(), e);
+        if (mode >= 3) {
+          double o6[6];
+          map_sums_fast(n, p1[c].data(), p2[c].data(), mu1[c].data(), mu2[c].data(), s11.data(),
+                        s22.data(), s12.data(), o6);
+          ssc[c * 6 + 0] = o6[0];
+          ssc[c * 6 + 1] = o6[1];
+          for (int k = 0; k < 4; ++k) e[k] = o6[2 + k];
+        } else {
+          ssim_sums_f32(n, mu1[c].data(), mu2[c].data(), s11.data(), s22.data(), s12.data(),
+                        &ssc[c * 6 + 0], &ssc[c * 6 + 1]);
+          edge_sums_f32(n, p1[c].data(), mu1[c].data(), p2[c].data(), mu2[c].data(), e);
+        }
       } else {
         ssim_sums_f64(n, mu1[c].data(), mu2[c].data(), s11.data(), s22.data(), s12.data(),
                       &ssc[c * 6 + 0], &ssc[c * 6 + 1]);

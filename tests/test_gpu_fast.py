@@ -1,0 +1,148 @@
+"""GPU tests of the batched fast form (ssim2_batch_*).
+
+Bars (stated per test): against the FP64 oracle |dscore| <= 0.05 (the published
+target of the FP16-storage form); against the oracle's restatement of the same
+arithmetic (shifted statistics, exact FIR, f32 maps) a few 1e-3; a pair scores
+bit-identically alone and inside any batch, since every tile is reduced in a
+fixed order whatever the batch holds.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FAST_SHIFT = 7        # oracle mode: same arithmetic as the fast form
+FAST_SHIFT_F16 = 8    # ... with XYB of scales >= 1 stored in binary16
+F64 = 1
+BAR_F64 = 0.05
+TOL_SAME_ARITH = 2e-3
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def s2():
+    import ssimulacra2_hip
+    return ssimulacra2_hip
+
+
+MIX = [
+    # kind, seed, w, h, step
+    (4, 101, 512, 384, 16),
+    (2, 102, 300, 200, 8),
+    (4, 103, 97, 150, 40),
+    (1, 104, 64, 64, 16),
+    (2, 105, 777, 555, 24),
+    (4, 106, 16, 16, 24),
+    (1, 107, 8, 8, 16),
+    (4, 108, 1000, 90, 12),
+    (2, 109, 123, 45, 32),
+    (4, 110, 256, 256, 4),
+]
+
+
+def _upload(s2, synth, cases):
+    host, dev = [], []
+    for kind, seed, w, h, step in cases:
+        r, d = synth.pair(kind, seed, w, h, step)
+        host.append((r, d))
+        dev.append((s2.DeviceImage.from_host(r), s2.DeviceImage.from_host(d)))
+    return host, dev
+
+
+@pytest.fixture(scope="module")
+def mix(s2, synth):
+    return _upload(s2, synth, MIX)
+
+
+def test_batch_matches_oracle_and_f64(s2, oracle, mix):
+    host, dev = mix
+    _say(f"[gpu] fast batch f32, {len(dev)} pairs")
+    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000, storage="f32") as b:
+        scores = b.compute(dev)
+    for (r, d), sc, case in zip(host, scores, MIX):
+        same = oracle.score_srgb(r, d, FAST_SHIFT)
+        f64 = oracle.score_srgb(r, d, F64)
+        _say(f"[gpu] {case[2]}x{case[3]}: gpu {sc:.6f} same-arith {same:.6f} f64 {f64:.6f}")
+        assert abs(sc - same) <= TOL_SAME_ARITH, (case, sc, same)
+        assert abs(sc - f64) <= BAR_F64, (case, sc, f64)
+
+
+def test_alone_equals_in_the_mix(s2, mix):
+    _, dev = mix
+    _say("[gpu] alone vs mix")
+    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000) as b:
+        together = b.compute(dev)
+        alone = [b.compute([p])[0] for p in dev]
+        shuffled_idx = list(reversed(range(len(dev))))
+        shuffled = b.compute([dev[i] for i in shuffled_idx])
+    np.testing.assert_array_equal(together, np.array(alone))
+    np.testing.assert_array_equal(together[shuffled_idx], shuffled)
+
+
+def test_identical_pairs_score_100(s2, synth):
+    r, _ = synth.pair(2, 7, 200, 120, 16)
+    a = s2.DeviceImage.from_host(r)
+    _say("[gpu] identical pair, fast form")
+    with s2.Ssim2Batch(max_pairs=4, max_pixels=1_000_000) as b:
+        assert b.compute([(a, a)])[0] == 100.0
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_reduced_storage(s2, oracle, mix, storage):
+    host, dev = mix
+    _say(f"[gpu] fast batch {storage}")
+    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000, storage=storage) as b:
+        scores = b.compute(dev)
+    worst = 0.0
+    for (r, d), sc, case in zip(host, scores, MIX):
+        f64 = oracle.score_srgb(r, d, F64)
+        worst = max(worst, abs(sc - f64))
+        if storage == "f16":
+            same = oracle.score_srgb(r, d, FAST_SHIFT_F16)
+            assert abs(sc - same) <= 5e-3, (case, sc, same)
+        # the published bar is for FP16 storage at the quantiser steps of the
+        # workloads (>= 4); bfloat16 keeps 8 mantissa bits and is reported only
+        if storage == "f16":
+            assert abs(sc - f64) <= BAR_F64, (case, sc, f64)
+    _say(f"[gpu] {storage}: worst |d| vs f64 {worst:.4f}")
+
+
+def test_autotune_keeps_scores(s2, mix):
+    _, dev = mix
+    _say("[gpu] autotune")
+    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000) as b:
+        before = b.compute(dev)
+        rep = b.autotune(dev, reps=2)
+        _say(rep)
+        cfg, mult = b.tiling()
+        assert 0 <= cfg < 4 and mult in (2, 4, 8)
+        after = b.compute(dev)
+        for t in range(4):
+            b.set_tiling(t, 2)
+            np.testing.assert_allclose(b.compute(dev), before, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(after, before, rtol=0, atol=1e-6)
+
+
+def test_batch_errors(s2):
+    a = s2.DeviceImage(64, 64)
+    c = s2.DeviceImage(64, 32)
+    with s2.Ssim2Batch(max_pairs=2, max_pixels=100_000) as b:
+        with pytest.raises(s2.Ssim2Error) as ei:
+            b.compute([(a, c)])
+        assert ei.value.status == "SSIM2_DIM_MISMATCH"
+        with pytest.raises(s2.Ssim2Error) as ei:
+            b.compute([(a, a)] * 3)
+        assert ei.value.status == "SSIM2_BATCH_SIZE"
+        big = s2.DeviceImage(512, 512)
+        with pytest.raises(s2.Ssim2Error) as ei:
+            b.compute([(big, big)])
+        assert ei.value.status == "SSIM2_BATCH_SIZE"
+        tiny = s2.DeviceImage(7, 64)
+        with pytest.raises(s2.Ssim2Error) as ei:
+            b.compute([(tiny, tiny)])
+        assert ei.value.status == "SSIM2_INVALID_SIZE"
+        # the object stays usable after refused submissions
+        assert b.compute([(a, a)])[0] == 100.0

@@ -37,6 +37,10 @@ ssim2_status hip_fail(hipError_t e, const char* what) {
 
 }  // namespace
 
+namespace ssim2 {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace ssim2
+
 struct ssim2_ctx {
   int w = 0, h = 0;
   PairPlan plan;
@@ -99,7 +103,7 @@ InputArgs make_inputs(const ssim2_image* ref, const ssim2_image* dis, int kind) 
 ssim2_status enqueue(ssim2_ctx* c, int sides, const InputArgs& in, hipStream_t s) {
   const PairPlan& p = c->plan;
   HIP_TRY(launch_pyramid(c->d_desc, c->d_items + p.off_pyr, p.n_pyr, sides, in, s));
-  HIP_TRY(launch_rows(c->d_desc, c->d_items + p.off_rows, p.n_rows, sides, s));
+  HIP_TRY(launch_rows(c->d_desc, c->d_items + p.off_rows[sides], p.n_rows[sides], sides, s));
   HIP_TRY(launch_cols(c->d_desc, c->d_items + p.off_cols, p.n_cols, sides, s));
   if (sides != kSidesRef) {
     HIP_TRY(launch_finalize(c->d_desc, 1, s));
@@ -391,7 +395,7 @@ ssim2_status ssim2_time_kernels(ssim2_ctx* c, const ssim2_image* ref, const ssim
     if ((e = hipEventRecord(ev[0], s)) != hipSuccess ||
         (e = launch_pyramid(c->d_desc, c->d_items + p.off_pyr, p.n_pyr, kSidesBoth, in, s)) != hipSuccess ||
         (e = hipEventRecord(ev[1], s)) != hipSuccess ||
-        (e = launch_rows(c->d_desc, c->d_items + p.off_rows, p.n_rows, kSidesBoth, s)) != hipSuccess ||
+        (e = launch_rows(c->d_desc, c->d_items + p.off_rows[kSidesBoth], p.n_rows[kSidesBoth], kSidesBoth, s)) != hipSuccess ||
         (e = hipEventRecord(ev[2], s)) != hipSuccess ||
         (e = launch_cols(c->d_desc, c->d_items + p.off_cols, p.n_cols, kSidesBoth, s)) != hipSuccess ||
         (e = hipEventRecord(ev[3], s)) != hipSuccess ||

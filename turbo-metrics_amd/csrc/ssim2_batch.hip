@@ -1,0 +1,444 @@
+// Batch engine of the fast form (include/ssim2hip.h, ssim2_batch_*): many
+// uneven sRGB pairs per launch. Host code: planning, staging, the captured
+// hipGraph of the three persistent kernels, and the tile-shape autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ssim2hip.h"
+#include "ssim2_internal.h"
+#include "ssim2_plan.h"
+
+using namespace ssim2;
+
+namespace {
+
+ssim2_status bfail(ssim2_status st, const std::string& msg) {
+  set_last_error(msg);
+  return st;
+}
+
+#define BHIP(expr)                                                                            \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess) return bfail(SSIM2_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+size_t storage_bytes(int storage) { return storage == kStorageF32 ? 4 : 2; }
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Plan one pair of the fast form. Returns false when the size is unsupported.
+struct FastPlan {
+  PairDesc d{};
+  int npyr = 0;
+  int ntiles = 0;
+  size_t arena = 0;  // bytes of XYB planes (scales >= 1)
+  int nparts = 0;
+};
+
+bool plan_fast(int w, int h, int storage, int cfg, size_t arena_off, int part_base, FastPlan* out) {
+  if (!size_supported(w, h)) return false;
+  FastPlan& p = *out;
+  p = FastPlan{};
+  PairDesc& d = p.d;
+  d.w = w;
+  d.h = h;
+  int sw[kMaxScales], sh[kMaxScales];
+  d.nscales = scale_dims(w, h, sw, sh);
+  const int TX = kTileCfgW[cfg], TY = kTileCfgH[cfg];
+  const size_t sb = storage_bytes(storage);
+  size_t off = arena_off;
+  int part = part_base, tile = 0;
+  for (int s = 0; s < d.nscales; ++s) {
+    ScaleDesc& S = d.sc[s];
+    S.w = sw[s];
+    S.h = sh[s];
+    S.pitch = (sw[s] + 15) / 16 * 16;
+    S.plane = (long long)S.pitch * S.h;
+    if (s > 0) {
+      S.xyb = (long long)off;
+      off += align256(6 * (size_t)S.plane * sb);
+    }
+    S.tiles_x = (sw[s] + TX - 1) / TX;
+    S.colblocks = S.tiles_x * ((sh[s] + TY - 1) / TY);
+    S.tile0 = tile;
+    tile += S.colblocks;
+    S.part = part;
+    part += 3 * S.colblocks;
+  }
+  p.npyr = ((w + kPyrTile - 1) / kPyrTile) * ((h + kPyrTile - 1) / kPyrTile);
+  p.ntiles = tile;
+  p.arena = off - arena_off;
+  p.nparts = part - part_base;
+  return true;
+}
+
+}  // namespace
+
+struct ssim2_batch {
+  int storage = kStorageF32;
+  int tile_cfg = 0;
+  int grid_mult = 4;
+  uint32_t max_pairs = 0;
+  uint64_t max_pixels = 0;
+  size_t arena_cap = 0, part_cap = 0;
+  unsigned char* arena = nullptr;
+  double* partials = nullptr;
+  double* results = nullptr;
+  PairDesc* d_desc = nullptr;
+  int* d_index = nullptr;  // [meta(4) | pyr prefix (max+1) | tile prefix (max+1)]
+  PairDesc* h_desc = nullptr;
+  int* h_index = nullptr;
+  double* h_results = nullptr;
+  hipEvent_t done = nullptr;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_cfg = -1, graph_mult = -1;
+  bool pending = false;
+  int last_n = 0;
+  int cus = 256;
+  size_t bytes = 0;
+  std::string tune_report;
+};
+
+namespace {
+
+int* meta_ptr(int* base) { return base; }
+int* pyr_prefix(ssim2_batch* b, int* base) { return base + 4; }
+int* tile_prefix(ssim2_batch* b, int* base) { return base + 4 + (b->max_pairs + 1); }
+
+void release(ssim2_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->arena) (void)hipFree(b->arena);
+  if (b->partials) (void)hipFree(b->partials);
+  if (b->results) (void)hipFree(b->results);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_index) (void)hipFree(b->d_index);
+  if (b->h_desc) (void)hipHostFree(b->h_desc);
+  if (b->h_index) (void)hipHostFree(b->h_index);
+  if (b->h_results) (void)hipHostFree(b->h_results);
+}
+
+ssim2_status enqueue_kernels(ssim2_batch* b, hipStream_t s, int cfg, int mult) {
+  const int* meta = meta_ptr(b->d_index);
+  BHIP(launch_fast_pyramid(b->storage, b->d_desc, pyr_prefix(b, b->d_index), meta, b->cus * 2, s));
+  BHIP(launch_fast_tile(b->storage, cfg, b->d_desc, tile_prefix(b, b->d_index), meta, b->cus * mult, s));
+  BHIP(launch_fast_finalize(b->d_desc, meta, std::min<int>(b->cus, (int)b->max_pairs), s));
+  return SSIM2_OK;
+}
+
+// Capture the three persistent launches once per (tile shape, grid size); the
+// grids are fixed and all sizes come from device memory, so the same graph
+// runs every batch.
+ssim2_status ensure_graph(ssim2_batch* b) {
+  if (b->graph && b->graph_cfg == b->tile_cfg && b->graph_mult == b->grid_mult) return SSIM2_OK;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  BHIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+  ssim2_status st = enqueue_kernels(b, b->cap_stream, b->tile_cfg, b->grid_mult);
+  hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+  if (st != SSIM2_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return bfail(SSIM2_HIP_ERROR, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->graph = nullptr;
+    return bfail(SSIM2_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  b->graph_cfg = b->tile_cfg;
+  b->graph_mult = b->grid_mult;
+  return SSIM2_OK;
+}
+
+ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg) {
+  if (n == 0) return bfail(SSIM2_BATCH_SIZE, "empty batch");
+  if (n > b->max_pairs)
+    return bfail(SSIM2_BATCH_SIZE, std::to_string(n) + " pairs exceed the batch capacity of " +
+                                       std::to_string(b->max_pairs));
+  if (b->pending) BHIP(hipEventSynchronize(b->done));  // staging and device buffers are free
+  b->pending = false;
+  size_t arena = 0;
+  int parts = 0, pyr = 0, tiles = 0;
+  uint64_t px = 0;
+  int* pp = pyr_prefix(b, b->h_index);
+  int* tp = tile_prefix(b, b->h_index);
+  for (uint32_t i = 0; i < n; ++i) {
+    const ssim2_pair& q = pairs[i];
+    if (!q.ref.ptr || !q.dis.ptr) return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": null image");
+    if (q.ref.width != q.dis.width || q.ref.height != q.dis.height)
+      return bfail(SSIM2_DIM_MISMATCH, "pair " + std::to_string(i) + ": reference and distorted sizes differ");
+    if (q.ref.pitch_bytes < size_t(q.ref.width) * 3 || q.dis.pitch_bytes < size_t(q.dis.width) * 3)
+      return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": pitch smaller than a row");
+    FastPlan fp;
+    if (!plan_fast((int)q.ref.width, (int)q.ref.height, b->storage, cfg, arena, parts, &fp))
+      return bfail(SSIM2_INVALID_SIZE, "pair " + std::to_string(i) + ": size " + std::to_string(q.ref.width) +
+                                           "x" + std::to_string(q.ref.height) + " unsupported");
+    px += uint64_t(q.ref.width) * q.ref.height;
+    PairDesc d = fp.d;
+    d.in_ref = static_cast<const unsigned char*>(q.ref.ptr);
+    d.in_dis = static_cast<const unsigned char*>(q.dis.ptr);
+    d.in_ref_pitch = (long long)q.ref.pitch_bytes;
+    d.in_dis_pitch = (long long)q.dis.pitch_bytes;
+    d.in_kind = kInputSrgbU8;
+    d.arena = reinterpret_cast<float*>(b->arena);
+    d.partials = b->partials;
+    d.result = b->results + size_t(i) * kResultDoubles;
+    b->h_desc[i] = d;
+    pp[i] = pyr;
+    tp[i] = tiles;
+    pyr += fp.npyr;
+    tiles += fp.ntiles;
+    arena += fp.arena;
+    parts += fp.nparts;
+  }
+  if (px > b->max_pixels || arena > b->arena_cap || size_t(parts) > b->part_cap)
+    return bfail(SSIM2_BATCH_SIZE, "batch of " + std::to_string(px) + " px exceeds the capacity of " +
+                                       std::to_string(b->max_pixels) + " px");
+  pp[n] = pyr;
+  tp[n] = tiles;
+  int* meta = meta_ptr(b->h_index);
+  meta[0] = (int)n;
+  meta[1] = pyr;
+  meta[2] = tiles;
+  meta[3] = 0;
+  return SSIM2_OK;
+}
+
+ssim2_status submit(ssim2_batch* b, uint32_t n, hipStream_t s, bool use_graph, int cfg, int mult) {
+  const size_t index_ints = 4 + 2 * size_t(b->max_pairs + 1);
+  BHIP(hipMemcpyAsync(b->d_desc, b->h_desc, n * sizeof(PairDesc), hipMemcpyHostToDevice, s));
+  BHIP(hipMemcpyAsync(b->d_index, b->h_index, index_ints * sizeof(int), hipMemcpyHostToDevice, s));
+  if (use_graph) {
+    ssim2_status st = ensure_graph(b);
+    if (st != SSIM2_OK) return st;
+    BHIP(hipGraphLaunch(b->graph, s));
+  } else {
+    ssim2_status st = enqueue_kernels(b, s, cfg, mult);
+    if (st != SSIM2_OK) return st;
+  }
+  BHIP(hipMemcpyAsync(b->h_results, b->results, n * kResultDoubles * sizeof(double), hipMemcpyDeviceToHost, s));
+  BHIP(hipEventRecord(b->done, s));
+  b->pending = true;
+  b->last_n = (int)n;
+  return SSIM2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ssim2_status ssim2_batch_create(const ssim2_batch_options* o, ssim2_batch** out) {
+  if (!out || !o) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (o->struct_size != sizeof(ssim2_batch_options))
+    return bfail(SSIM2_INVALID_ARGUMENT, "ssim2_batch_options.struct_size mismatch");
+  if (o->storage > SSIM2_STORAGE_BF16) return bfail(SSIM2_INVALID_ARGUMENT, "unknown storage type");
+  if (o->max_pairs == 0 || o->max_pairs > (1u << 20) || o->max_pixels < 64)
+    return bfail(SSIM2_INVALID_ARGUMENT, "max_pairs must be in [1, 2^20] and max_pixels >= 64");
+  ssim2_batch* b = new (std::nothrow) ssim2_batch();
+  if (!b) return bfail(SSIM2_OOM, "host allocation failed");
+  b->storage = (int)o->storage;
+  b->max_pairs = o->max_pairs;
+  b->max_pixels = o->max_pixels;
+  const size_t sb = storage_bytes(b->storage);
+  // Scales >= 1 hold at most ~1/3 of the scale-0 pixels; pitch rounding adds
+  // at most 15 columns per plane row. A batch whose exact need exceeds this
+  // capacity is refused with SSIM2_BATCH_SIZE (split it).
+  b->arena_cap = 6 * sb * (size_t(o->max_pixels) / 2 + size_t(o->max_pairs) * 32768) +
+                 size_t(256) * kMaxScales * o->max_pairs;
+  b->part_cap = size_t(o->max_pixels) * 3 / 128 + size_t(o->max_pairs) * 3 * 2048;
+  const size_t index_ints = 4 + 2 * size_t(o->max_pairs + 1);
+  b->bytes = b->arena_cap + b->part_cap * kNumSums * sizeof(double) +
+             size_t(o->max_pairs) * (kResultDoubles * sizeof(double) + sizeof(PairDesc)) + index_ints * sizeof(int);
+  size_t budget = (size_t)o->memory_budget;
+  if (budget == 0) {
+    size_t fb = 0, tb = 0;
+    if (hipMemGetInfo(&fb, &tb) == hipSuccess) budget = fb;
+  }
+  if (budget && b->bytes > budget) {
+    const std::string m = "batch footprint " + std::to_string(b->bytes) + " B exceeds the budget of " +
+                          std::to_string(budget) + " B";
+    delete b;
+    return bfail(SSIM2_OOM, m);
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) b->cus = v;
+  }
+  hipError_t e;
+  if ((e = hipMalloc(&b->arena, b->arena_cap)) != hipSuccess ||
+      (e = hipMalloc(&b->partials, b->part_cap * kNumSums * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&b->results, size_t(o->max_pairs) * kResultDoubles * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&b->d_desc, size_t(o->max_pairs) * sizeof(PairDesc))) != hipSuccess ||
+      (e = hipMalloc(&b->d_index, index_ints * sizeof(int))) != hipSuccess ||
+      (e = hipHostMalloc(&b->h_desc, size_t(o->max_pairs) * sizeof(PairDesc), hipHostMallocDefault)) != hipSuccess ||
+      (e = hipHostMalloc(&b->h_index, index_ints * sizeof(int), hipHostMallocDefault)) != hipSuccess ||
+      (e = hipHostMalloc(&b->h_results, size_t(o->max_pairs) * kResultDoubles * sizeof(double),
+                         hipHostMallocDefault)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess) {
+    release(b);
+    delete b;
+    (void)hipGetLastError();
+    return bfail(SSIM2_OOM, std::string("batch allocation failed: ") + hipGetErrorString(e));
+  }
+  std::memset(b->h_index, 0, index_ints * sizeof(int));
+  *out = b;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_destroy(ssim2_batch* b) {
+  if (!b) return SSIM2_OK;
+  if (b->pending) (void)hipEventSynchronize(b->done);
+  release(b);
+  delete b;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_mem_usage(const ssim2_batch* b, size_t* bytes) {
+  if (!b || !bytes) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  *bytes = b->bytes;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_compute_async(ssim2_batch* b, const ssim2_pair* pairs, uint32_t npairs,
+                                       ssim2_stream stream) {
+  if (!b || (!pairs && npairs)) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st = stage(b, pairs, npairs, b->tile_cfg);
+  if (st != SSIM2_OK) return st;
+  return submit(b, npairs, (hipStream_t)stream, true, b->tile_cfg, b->grid_mult);
+}
+
+ssim2_status ssim2_batch_get_scores(ssim2_batch* b, double* scores, uint32_t capacity) {
+  if (!b || !scores) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (!b->pending && b->last_n == 0) return bfail(SSIM2_INVALID_ARGUMENT, "no batch has been submitted");
+  if (capacity < (uint32_t)b->last_n) return bfail(SSIM2_BATCH_SIZE, "score buffer too small");
+  if (b->pending) BHIP(hipEventSynchronize(b->done));
+  b->pending = false;
+  for (int i = 0; i < b->last_n; ++i) scores[i] = b->h_results[size_t(i) * kResultDoubles];
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_query(ssim2_batch* b, int* done) {
+  if (!b || !done) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (!b->pending) {
+    *done = 1;
+    return SSIM2_OK;
+  }
+  hipError_t e = hipEventQuery(b->done);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return SSIM2_OK;
+  }
+  BHIP(e);
+  *done = 1;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_get_sums(ssim2_batch* b, uint32_t index, double* sums108, int* dims12) {
+  if (!b) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (index >= (uint32_t)b->last_n) return bfail(SSIM2_INVALID_ARGUMENT, "pair index out of range");
+  if (b->pending) BHIP(hipEventSynchronize(b->done));
+  b->pending = false;
+  const double* r = b->h_results + size_t(index) * kResultDoubles;
+  if (sums108) std::memcpy(sums108, r + 1, kSumsPerPair * sizeof(double));
+  if (dims12)
+    for (int i = 0; i < 2 * kMaxScales; ++i) dims12[i] = (int)r[1 + kSumsPerPair + i];
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_compute_sync(ssim2_batch* b, const ssim2_pair* pairs, uint32_t npairs,
+                                      ssim2_stream stream, double* scores) {
+  ssim2_status st = ssim2_batch_compute_async(b, pairs, npairs, stream);
+  if (st != SSIM2_OK) return st;
+  return ssim2_batch_get_scores(b, scores, npairs);
+}
+
+ssim2_status ssim2_batch_set_tiling(ssim2_batch* b, uint32_t tile_cfg, uint32_t grid_mult) {
+  if (!b || tile_cfg >= (uint32_t)kNumTileCfgs || grid_mult == 0 || grid_mult > 64)
+    return bfail(SSIM2_INVALID_ARGUMENT, "tile_cfg in [0, 3], grid_mult in [1, 64]");
+  b->tile_cfg = (int)tile_cfg;
+  b->grid_mult = (int)grid_mult;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_get_tiling(const ssim2_batch* b, uint32_t* tile_cfg, uint32_t* grid_mult) {
+  if (!b) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (tile_cfg) *tile_cfg = (uint32_t)b->tile_cfg;
+  if (grid_mult) *grid_mult = (uint32_t)b->grid_mult;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint32_t npairs,
+                                  ssim2_stream stream, uint32_t reps, char* report, size_t report_size) {
+  if (!b || !pairs || npairs == 0) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (reps == 0) reps = 3;
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  BHIP(hipEventCreate(&e0));
+  BHIP(hipEventCreate(&e1));
+  static const int kMults[] = {2, 4, 8};
+  float best = 1e30f;
+  int best_cfg = b->tile_cfg, best_mult = b->grid_mult;
+  std::string rep = "tile     grid_x_cu  ms/batch\n";
+  ssim2_status st = SSIM2_OK;
+  for (int cfg = 0; cfg < kNumTileCfgs && st == SSIM2_OK; ++cfg)
+    for (int mult : kMults) {
+      if ((st = stage(b, pairs, npairs, cfg)) != SSIM2_OK) break;
+      if ((st = submit(b, npairs, s, false, cfg, mult)) != SSIM2_OK) break;  // warm-up
+      hipError_t e = hipEventRecord(e0, s);
+      for (uint32_t r = 0; r < reps && e == hipSuccess; ++r)
+        if (enqueue_kernels(b, s, cfg, mult) != SSIM2_OK) e = hipErrorUnknown;
+      if (e == hipSuccess) e = hipEventRecord(e1, s);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        st = bfail(SSIM2_HIP_ERROR, std::string("autotune: ") + hipGetErrorString(e));
+        break;
+      }
+      ms /= reps;
+      char line[96];
+      std::snprintf(line, sizeof line, "%2dx%-5d %-10d %.4f\n", kTileCfgW[cfg], kTileCfgH[cfg], mult, ms);
+      rep += line;
+      if (ms < best) {
+        best = ms;
+        best_cfg = cfg;
+        best_mult = mult;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->pending = false;
+  if (st != SSIM2_OK) return st;
+  BHIP(hipStreamSynchronize(s));
+  b->tile_cfg = best_cfg;
+  b->grid_mult = best_mult;
+  char line[96];
+  std::snprintf(line, sizeof line, "chosen %dx%d, grid %d x CUs\n", kTileCfgW[best_cfg], kTileCfgH[best_cfg], best_mult);
+  rep += line;
+  b->tune_report = rep;
+  if (report && report_size) {
+    std::strncpy(report, rep.c_str(), report_size - 1);
+    report[report_size - 1] = 0;
+  }
+  return SSIM2_OK;
+}
+
+}  // extern "C"

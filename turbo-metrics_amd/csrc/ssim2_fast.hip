@@ -1,0 +1,478 @@
+// ssimulacra2 fast form for gfx950: batched, persistent, FP32/FP16/BF16 storage.
+//
+// Same metric as the exact path, different arithmetic, chosen for throughput:
+//  * the blur is the 9-tap FIR that the reference's recursive Gaussian equals
+//    in exact arithmetic (crates/ssimulacra2-cuda-kernel/build.rs:28-140; the
+//    identity is checked in tests/test_oracle.py), evaluated as a separable
+//    tile convolution in LDS instead of a sequential scan;
+//  * the five blurred quantities are formed in shifted coordinates
+//    x' = x - shift(channel); the statistics are rebuilt with the exact
+//    corrections for the zero-padded border (b1 = blur of ones, e = 1 - b1):
+//        var = blur(x'^2) - blur(x')^2 + e*c*(2*blur(x') + c*b1)
+//        cov = blur(r'd') - blur(r')*blur(d') + e*c*(blur(r') + blur(d') + c*b1)
+//        x - mu = (x' - blur(x')) + c*e,  mu1 - mu2 = blur(r') - blur(d')
+//    which removes most f32 cancellation in sigma - mu^2;
+//  * the cube root is exp2(log2(x)/3) plus one Newton step (f32), the error
+//    maps are f32 (L1/L4 sums per block in f64);
+//  * XYB of scales >= 1 is stored in HBM as (value - shift) in the storage type
+//    (f32, IEEE half or bfloat16); scale 0 is recomputed from the u8 input
+//    inside each tile.
+//
+// Launches (all persistent: a fixed grid loops over a device-resident item
+// list, so one captured hipGraph serves every batch):
+//   k_fast_pyramid  32x32 scale-0 tiles of every pair: linear pyramid in LDS,
+//                   shifted XYB planes of scales >= 1.
+//   k_fast_tile     one TXxTY output tile of one scale of one pair per item:
+//                   XYB tile + 4 px halo -> products -> horizontal FIR ->
+//                   vertical FIR -> maps -> f64 block sums.
+//   k_fast_finalize one pair per item: fixed-order reduction and score.
+#include "ssim2_internal.h"
+#include "ssim2_math.h"
+#include "ssim2_tables.h"
+
+namespace ssim2 {
+
+namespace {
+
+__constant__ float c_lut_fast[256] = SSIM2_SRGB8_LUT_INIT;
+__constant__ double c_weight_fast[108] = SSIM2_WEIGHTS_INIT;
+
+// 9-tap FIR (centre first), the double taps of the recursive Gaussian rounded to f32.
+constexpr float kH0 = 0x1.0ef8d6p-2f, kH1 = 0x1.b413e8p-3f, kH2 = 0x1.bfd672p-4f,
+                kH3 = 0x1.2700c8p-5f, kH4 = 0x1.347d70p-7f;
+
+// Channel shifts: centre of the sRGB gamut in positive XYB (X 0.205..0.813,
+// Y 0.010..0.855, B 0.272..0.938), exactly representable.
+__device__ __forceinline__ float xyb_shift(int c) { return c == 0 ? 0.5f : (c == 1 ? 0.4375f : 0.625f); }
+
+__device__ __forceinline__ float cbrt_fast(float x) {
+  if (!(x > 0.0f)) return 0.0f;
+  float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * 0.333333343f);
+  const float r = __builtin_amdgcn_rcpf(y * y);
+  return fmaf(fmaf(x, r, -y), 0.333333343f, y);
+}
+
+__device__ __forceinline__ void xyb_fast(float r, float g, float b, float& X, float& Y, float& B) {
+  float rg = fmaf(kM00, r, fmaf(kM01, g, fmaf(kM02, b, kB0)));
+  float gr = fmaf(kM10, r, fmaf(kM11, g, fmaf(kM12, b, kB0)));
+  float bb = fmaf(kM20, r, fmaf(kM21, g, fmaf(kM22, b, kB0)));
+  rg = cbrt_fast(rg) - kB0Root;
+  gr = cbrt_fast(gr) - kB0Root;
+  bb = cbrt_fast(bb) - kB0Root;
+  const float x = 0.5f * (rg - gr), y = 0.5f * (rg + gr);
+  X = fmaf(x, 14.0f, 0.42f);
+  Y = y + 0.01f;
+  B = (bb - y) + 0.55f;
+}
+
+// Storage codecs.
+template <typename T>
+struct Codec;
+template <>
+struct Codec<float> {
+  __device__ static float enc(float v) { return v; }
+  __device__ static float dec(float v) { return v; }
+};
+template <>
+struct Codec<_Float16> {
+  __device__ static _Float16 enc(float v) { return (_Float16)v; }
+  __device__ static float dec(_Float16 v) { return (float)v; }
+};
+struct bf16_t {
+  unsigned short bits;
+};
+template <>
+struct Codec<bf16_t> {
+  __device__ static bf16_t enc(float v) {  // round to nearest even (finite inputs)
+    const unsigned u = __float_as_uint(v);
+    return bf16_t{(unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16)};
+  }
+  __device__ static float dec(bf16_t v) { return __uint_as_float((unsigned)v.bits << 16); }
+};
+
+__device__ __forceinline__ void fast_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename T>
+__device__ __forceinline__ T* fast_plane(const PairDesc& P, int s, int img, int c) {
+  const ScaleDesc& S = P.sc[s];
+  return reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(P.arena) + S.xyb) +
+         (long long)(img * 3 + c) * S.plane;
+}
+
+// Sum of the in-range taps of the 1-D FIR at position g of a line of n (the
+// 1-D blur of ones); exactly 1 away from the border.
+__device__ __forceinline__ float tap_sum(int g, int n) {
+  if (g >= 4 && g < n - 4) return 1.0f;
+  const float h[5] = {kH0, kH1, kH2, kH3, kH4};
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 4; k >= 1; --k) {
+    acc += (g - k >= 0) ? h[k] : 0.0f;
+    acc += (g + k < n) ? h[k] : 0.0f;
+  }
+  return acc + h[0];
+}
+
+// Index of the pair that owns flat item idx: prefix[p] is the first item of
+// pair p (prefix[0] = 0, nondecreasing).
+__device__ __forceinline__ int find_pair(const int* __restrict__ prefix, int npairs, int idx) {
+  int lo = 0, hi = npairs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= idx)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- pyramid
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __restrict__ pairs,
+                                                           const int* __restrict__ prefix,
+                                                           const int* __restrict__ meta) {
+  constexpr int T0 = kPyrTile;
+  __shared__ float bufA[6][T0][T0 + 1];
+  __shared__ float bufB[6][T0 / 2][T0 / 2 + 1];
+  const int npairs = meta[0], n = prefix[npairs];
+  const int tid = threadIdx.x;
+  for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+    const int pi = find_pair(prefix, npairs, idx);
+    const PairDesc& P = pairs[pi];
+    const int local = idx - prefix[pi];
+    const int ntx = (P.w + T0 - 1) / T0;
+    const int tx = local % ntx, ty = local / ntx;
+    fast_barrier();  // previous item's LDS reads are done
+    for (int p = tid; p < T0 * T0; p += kThreads) {
+      const int lx = p % T0, ly = p / T0;
+      const int gx = tx * T0 + lx, gy = ty * T0 + ly;
+      if (gx >= P.w || gy >= P.h) continue;
+#pragma unroll
+      for (int img = 0; img < 2; ++img) {
+        const unsigned char* px = (img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = c_lut_fast[px[c]];
+      }
+    }
+    for (int s = 1; s < P.nscales; ++s) {
+      fast_barrier();
+      const int Ts = T0 >> s;
+      const int pw = P.sc[s - 1].w, ph = P.sc[s - 1].h;
+      const int cw = P.sc[s].w, ch = P.sc[s].h;
+      const int ox0 = tx * Ts, oy0 = ty * Ts;
+      for (int p = tid; p < Ts * Ts; p += kThreads) {
+        const int lx = p % Ts, ly = p / Ts;
+        const int gx = ox0 + lx, gy = oy0 + ly;
+        if (gx >= cw || gy >= ch) continue;
+        const int x0 = min(2 * gx, pw - 1) - 2 * ox0, x1 = min(2 * gx + 1, pw - 1) - 2 * ox0;
+        const int y0 = min(2 * gy, ph - 1) - 2 * oy0, y1 = min(2 * gy + 1, ph - 1) - 2 * oy0;
+#pragma unroll
+        for (int img = 0; img < 2; ++img) {
+          float rgb[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int k = img * 3 + c;
+            float sum;
+            if (s & 1)
+              sum = ((bufA[k][y0][x0] + bufA[k][y0][x1]) + bufA[k][y1][x0]) + bufA[k][y1][x1];
+            else
+              sum = ((bufB[k][y0][x0] + bufB[k][y0][x1]) + bufB[k][y1][x0]) + bufB[k][y1][x1];
+            rgb[c] = sum * 0.25f;
+            if (s & 1)
+              bufB[k][ly][lx] = rgb[c];
+            else
+              bufA[k][ly][lx] = rgb[c];
+          }
+          float X, Y, B;
+          xyb_fast(rgb[0], rgb[1], rgb[2], X, Y, B);
+          const long long off = (long long)gy * P.sc[s].pitch + gx;
+          fast_plane<T>(P, s, img, 0)[off] = Codec<T>::enc(X - xyb_shift(0));
+          fast_plane<T>(P, s, img, 1)[off] = Codec<T>::enc(Y - xyb_shift(1));
+          fast_plane<T>(P, s, img, 2)[off] = Codec<T>::enc(B - xyb_shift(2));
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- fused tile
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restrict__ pairs,
+                                                        const int* __restrict__ prefix,
+                                                        const int* __restrict__ meta) {
+  static_assert(TX % 4 == 0 && TY % 2 == 0, "tile shape");
+  constexpr int IW = TX + 8, IH = TY + 8;
+  __shared__ float xs[2][3][IH][IW + 1];  // shifted XYB, zero outside the image
+  __shared__ float hq[5][IH][TX + 1];     // horizontal pass, one channel at a time
+  __shared__ double red[kThreads / 64][18];
+  const int npairs = meta[0], n = prefix[npairs];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+    const int pi = find_pair(prefix, npairs, idx);
+    const PairDesc& P = pairs[pi];
+    int s = 0, tile = idx - prefix[pi];
+    while (s + 1 < P.nscales && tile >= P.sc[s + 1].tile0) ++s;
+    tile -= P.sc[s].tile0;
+    const ScaleDesc& S = P.sc[s];
+    const int w = S.w, h = S.h;
+    const int tx = tile % S.tiles_x, ty = tile / S.tiles_x;
+    const int x0 = tx * TX, y0 = ty * TY;
+    fast_barrier();  // previous item's LDS reads are done
+
+    // Phase 1: shifted XYB of the tile and its 4 px halo.
+    for (int e = tid; e < IH * IW; e += kThreads) {
+      const int iy = e / IW, ix = e % IW;
+      const int gx = x0 - 4 + ix, gy = y0 - 4 + iy;
+      float v[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+      if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+        if (s == 0) {
+#pragma unroll
+          for (int img = 0; img < 2; ++img) {
+            const unsigned char* px =
+                (img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
+            float X, Y, B;
+            xyb_fast(c_lut_fast[px[0]], c_lut_fast[px[1]], c_lut_fast[px[2]], X, Y, B);
+            v[img][0] = X - xyb_shift(0);
+            v[img][1] = Y - xyb_shift(1);
+            v[img][2] = B - xyb_shift(2);
+          }
+        } else {
+          const long long off = (long long)gy * S.pitch + gx;
+#pragma unroll
+          for (int img = 0; img < 2; ++img)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[img][c] = Codec<T>::dec(fast_plane<T>(P, s, img, c)[off]);
+        }
+      }
+#pragma unroll
+      for (int img = 0; img < 2; ++img)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xs[img][c][iy][ix] = v[img][c];
+    }
+
+    float acc[3][6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc[c][j] = 0.0f;
+
+    for (int c = 0; c < 3; ++c) {
+      fast_barrier();  // xs complete (c = 0) / previous channel's hq reads done
+      // Phase 2: horizontal FIR of the five quantities, 4 outputs per unit.
+      for (int e = tid; e < IH * (TX / 4); e += kThreads) {
+        const int iy = e / (TX / 4), ox0 = (e % (TX / 4)) * 4;
+        float a[12], d[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          a[k] = xs[0][c][iy][ox0 + k];
+          d[k] = xs[1][c][iy][ox0 + k];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          float qv[12];
+#pragma unroll
+          for (int k = 0; k < 12; ++k)
+            qv[k] = q == 0 ? a[k] * a[k] : (q == 1 ? d[k] * d[k] : (q == 2 ? a[k] * d[k] : (q == 3 ? a[k] : d[k])));
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            float r = kH4 * (qv[j] + qv[j + 8]);
+            r = fmaf(kH3, qv[j + 1] + qv[j + 7], r);
+            r = fmaf(kH2, qv[j + 2] + qv[j + 6], r);
+            r = fmaf(kH1, qv[j + 3] + qv[j + 5], r);
+            r = fmaf(kH0, qv[j + 4], r);
+            hq[q][iy][ox0 + j] = r;
+          }
+        }
+      }
+      fast_barrier();
+      // Phase 3: vertical FIR (two rows per unit), statistics, error maps.
+      const float sh = xyb_shift(c);
+      for (int e = tid; e < (TY / 2) * TX; e += kThreads) {
+        const int oy0 = (e / TX) * 2, ox = e % TX;
+        float bq[5][2];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          float col[10];
+#pragma unroll
+          for (int k = 0; k < 10; ++k) col[k] = hq[q][oy0 + k][ox];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            float r = kH4 * (col[j] + col[j + 8]);
+            r = fmaf(kH3, col[j + 1] + col[j + 7], r);
+            r = fmaf(kH2, col[j + 2] + col[j + 6], r);
+            r = fmaf(kH1, col[j + 3] + col[j + 5], r);
+            r = fmaf(kH0, col[j + 4], r);
+            bq[q][j] = r;
+          }
+        }
+        const int gx = x0 + ox;
+        const float b1x = tap_sum(gx, w);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int gy = y0 + oy0 + j;
+          if (gx >= w || gy >= h) continue;
+          const float b1 = b1x * tap_sum(gy, h);
+          const float ec = (1.0f - b1) * sh, cb = sh * b1;
+          const float s11 = bq[0][j], s22 = bq[1][j], s12 = bq[2][j], m1 = bq[3][j], m2 = bq[4][j];
+          const float var1 = (s11 - m1 * m1) + ec * (2.0f * m1 + cb);
+          const float var2 = (s22 - m2 * m2) + ec * (2.0f * m2 + cb);
+          const float cov = (s12 - m1 * m2) + ec * ((m1 + m2) + cb);
+          const float md = m1 - m2;
+          const float num_m = fmaf(md, -md, 1.0f);
+          const float num_s = fmaf(2.0f, cov, kC2);
+          const float den = (var1 + var2) + kC2;
+          const float dd = fmaxf(1.0f - (num_m * num_s) / den, 0.0f);
+          const float d2 = dd * dd;
+          acc[c][0] += dd;
+          acc[c][1] += d2 * d2;
+          const float src = xs[0][c][oy0 + j + 4][ox + 4], dis = xs[1][c][oy0 + j + 4][ox + 4];
+          const float d1 = (1.0f + fabsf((dis - m2) + ec)) / (1.0f + fabsf((src - m1) + ec)) - 1.0f;
+          const float art = fmaxf(d1, 0.0f), det = fmaxf(-d1, 0.0f);
+          const float a2 = art * art, e2 = det * det;
+          acc[c][2] += art;
+          acc[c][3] += a2 * a2;
+          acc[c][4] += det;
+          acc[c][5] += e2 * e2;
+        }
+      }
+    }
+    // Block sums in f64, fixed order: wave shuffles, then waves in sequence.
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double v = wave_sum_d((double)acc[c][j]);
+        if (lane == 0) red[wv][c * 6 + j] = v;
+      }
+    fast_barrier();
+    if (tid < 18) {
+      double v = 0.0;
+      for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
+      const int c = tid / 6, j = tid % 6;
+      P.partials[(long long)(S.part + c * S.colblocks + tile) * kNumSums + j] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- finalize
+
+__global__ __launch_bounds__(kThreads) void k_fast_finalize(const PairDesc* __restrict__ pairs,
+                                                            const int* __restrict__ meta) {
+  __shared__ double sums[kSumsPerPair];
+  const int n = meta[0];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int pi = blockIdx.x; pi < n; pi += gridDim.x) {
+    const PairDesc& P = pairs[pi];
+    fast_barrier();
+    for (int g = wv; g < kMaxScales * 3; g += kThreads / 64) {
+      const int s = g / 3, c = g % 3;
+      double v[kNumSums] = {0, 0, 0, 0, 0, 0};
+      if (s < P.nscales) {
+        const ScaleDesc& S = P.sc[s];
+        const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums;
+        for (int b = lane; b < S.colblocks; b += 64)
+#pragma unroll
+          for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
+      }
+#pragma unroll
+      for (int q = 0; q < kNumSums; ++q) {
+        const double r = wave_sum_d(v[q]);
+        if (lane == 0) {
+          sums[g * kNumSums + q] = r;
+          P.result[1 + g * kNumSums + q] = r;
+        }
+      }
+    }
+    if (t < 2 * kMaxScales) {
+      const int s = t / 2;
+      P.result[1 + kSumsPerPair + t] = s < P.nscales ? (double)(t & 1 ? P.sc[s].h : P.sc[s].w) : 0.0;
+    }
+    fast_barrier();
+    if (t == 0) {
+      double ssim = 0.0;
+      int i = 0;
+      for (int c = 0; c < 3; ++c)
+        for (int s = 0; s < P.nscales; ++s) {
+          const double opp = 1.0 / (double)((long long)P.sc[s].w * P.sc[s].h);
+          const double* q = sums + s * 18 + c * kNumSums;
+          const double l1[3] = {opp * q[0], opp * q[2], opp * q[4]};
+          const double l4[3] = {sqrt(sqrt(opp * q[1])), sqrt(sqrt(opp * q[3])), sqrt(sqrt(opp * q[5]))};
+          for (int m = 0; m < 3; ++m) ssim = fma(c_weight_fast[i++], fabs(l1[m]), ssim);
+          for (int m = 0; m < 3; ++m) ssim = fma(c_weight_fast[i++], fabs(l4[m]), ssim);
+        }
+      ssim *= 0.9562382616834844;
+      ssim = fma(6.248496625763138e-5 * ssim * ssim, ssim,
+                 fma(2.326765642916932, ssim, -0.020884521182843837 * ssim * ssim));
+      ssim = ssim > 0.0 ? fma(pow(ssim, 0.6276336467831387), -10.0, 100.0) : 100.0;
+      P.result[0] = ssim;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta,
+                               int grid, hipStream_t stream) {
+  switch (storage) {
+    case kStorageF32:
+      hipLaunchKernelGGL(k_fast_pyramid<float>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+    case kStorageF16:
+      hipLaunchKernelGGL(k_fast_pyramid<_Float16>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+    default:
+      hipLaunchKernelGGL(k_fast_pyramid<bf16_t>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_tile_t(int tile_cfg, const PairDesc* pairs, const int* prefix, const int* meta,
+                                int grid, hipStream_t stream) {
+  switch (tile_cfg) {
+    case 0:
+      hipLaunchKernelGGL((k_fast_tile<T, 32, 16>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_fast_tile<T, 64, 16>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_fast_tile<T, 32, 32>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fast_tile<T, 16, 16>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, const int* prefix,
+                            const int* meta, int grid, hipStream_t stream) {
+  switch (storage) {
+    case kStorageF32: return launch_tile_t<float>(tile_cfg, pairs, prefix, meta, grid, stream);
+    case kStorageF16: return launch_tile_t<_Float16>(tile_cfg, pairs, prefix, meta, grid, stream);
+    default: return launch_tile_t<bf16_t>(tile_cfg, pairs, prefix, meta, grid, stream);
+  }
+}
+
+hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(k_fast_finalize, dim3(grid), dim3(kThreads), 0, stream, pairs, meta);
+  return hipGetLastError();
+}
+
+}  // namespace ssim2

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace ssim2 {
 
 constexpr int kMaxScales = 6;
@@ -18,11 +20,21 @@ constexpr int kSumsPerPair = kMaxScales * 3 * kNumSums;  // 108
 
 // Tile / block geometry (compile-time; the launch builders use the same values).
 constexpr int kPyrTile = 32;      // scale-0 tile edge of the pyramid kernel
-constexpr int kRowBlock = 16;     // rows per row-pass block
-constexpr int kRowChunk = 32;     // columns scanned per LDS chunk (row pass)
+constexpr int kRowsPerWave = 21;  // rows per row-pass wave (lanes 3r+k, 63 of 64 used)
 constexpr int kColBlock = 16;     // columns per column-pass block
-constexpr int kColChunk = 32;     // rows scanned per LDS chunk (column pass)
+constexpr int kColChunk = 16;     // rows per column-pass chunk (one pixel per thread)
 constexpr int kThreads = 256;
+
+enum Storage : int {
+  kStorageF32 = 0,
+  kStorageF16 = 1,
+  kStorageBF16 = 2,
+};
+
+// Output tile shapes of the fast form (k_fast_tile instantiations).
+constexpr int kNumTileCfgs = 4;
+constexpr int kTileCfgW[kNumTileCfgs] = {32, 64, 32, 16};
+constexpr int kTileCfgH[kNumTileCfgs] = {16, 16, 32, 16};
 
 enum InputKind : int {
   kInputSrgbU8 = 0,     // packed RGB u8, pitched
@@ -46,6 +58,8 @@ struct ScaleDesc {
   long long cache;   // element offset of the 9 cached-reference planes: ref XYB c, blurred
                      // ref^2 (sigma11) c, blurred ref (mu1) c, for c = 0..2
   int part;          // index of the first partial-sum record of this scale
+  int tiles_x;       // fast form: output tiles per row of tiles
+  int tile0;         // fast form: index of this scale's first tile within the pair
   int pad;
 };
 
@@ -85,6 +99,9 @@ struct WorkItem {
   int pair, a, b, c;
 };
 
+// Records the message returned by ssim2_last_error() on this thread.
+void set_last_error(const std::string& msg);
+
 // Launchers (kernels in ssim2_kernels.hip). `pairs` and `items` are device
 // pointers; `nitems` blocks are launched.
 hipError_t launch_pyramid(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
@@ -94,6 +111,13 @@ hipError_t launch_rows(const PairDesc* pairs, const WorkItem* items, int nitems,
 hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
                        hipStream_t stream);
 hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream);
+// Fast form (ssim2_fast.hip). `prefix` holds npairs+1 first-item indices,
+// `meta[0]` the pair count; grids are fixed (persistent kernels).
+hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta,
+                               int grid, hipStream_t stream);
+hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, const int* prefix,
+                            const int* meta, int grid, hipStream_t stream);
+hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream);
 hipError_t launch_srgb_to_linear(const unsigned char* in, long long in_pitch, float* out,
                                  long long out_pitch, int w, int h, hipStream_t stream);

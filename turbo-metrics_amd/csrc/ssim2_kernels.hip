@@ -55,6 +55,30 @@ struct SideTraits<kSidesDis> {
   __device__ static int q(int i) { return i == 0 ? 1 : (i == 1 ? 2 : 4); }
 };
 
+// Global-address-space views of plane pointers. Pointers read out of a
+// descriptor in memory are generic, and generic (flat) accesses may complete
+// out of order, which makes the compiler wait for every outstanding load
+// before using any one of them. Global loads complete in order, so a
+// register prefetch queue only waits for the oldest entry.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const float gcfloat;
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) vf4 gfloat4;
+typedef __attribute__((address_space(1))) const vf4 gcfloat4;
+__device__ __forceinline__ float4 to_f4(vf4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+__device__ __forceinline__ gcfloat* gbl(const float* p) { return (gcfloat*)p; }
+__device__ __forceinline__ gfloat* gbl(float* p) { return (gfloat*)p; }
+
+// Workgroup barrier that orders LDS only. __syncthreads() also waits for every
+// outstanding global load and store (vmcnt(0)), which would drain the
+// register prefetch queues at each barrier; LDS visibility needs only
+// lgkmcnt(0) before s_barrier.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ float quantity(int q, float a, float b) {
   switch (q) {
     case 0: return a * a;
@@ -143,7 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 
   // Scales 1..n-1: 2x2 mean of the previous scale (cpu.rs:545-579), then XYB.
   for (int s = 1; s < P.nscales; ++s) {
-    __syncthreads();
+    lds_barrier();
     const int T = T0 >> s;
     const int pw = P.sc[s - 1].w, ph = P.sc[s - 1].h;
     const int cw = P.sc[s].w, ch = P.sc[s].h;
@@ -192,70 +216,129 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 
 // ---------------------------------------------------------------- row pass
 
-template <int SIDES>
-__global__ __launch_bounds__(kThreads) void k_rows(const PairDesc* __restrict__ pairs,
-                                                   const WorkItem* __restrict__ items) {
-  constexpr int NQ = SideTraits<SIDES>::NQ;
-  constexpr int RB = kRowBlock, CX = kRowChunk, HALO = 2 * kRadius;  // left n-6, right n+4
-  constexpr int TW = CX + HALO;                                       // staged columns
-  __shared__ float Qs[NQ][RB][TW + 1];
-  __shared__ float Os[NQ * 3][RB][CX + 1];
-  const WorkItem it = items[blockIdx.x];
-  const PairDesc& P = pairs[it.pair];
-  const int s = it.a, c = it.b, y0 = it.c * RB;
-  const ScaleDesc& S = P.sc[s];
-  const int w = S.w, h = S.h, pitch = S.pitch;
-  const float* __restrict__ ra = xyb_plane(P, s, 0, c, SIDES);
-  const float* __restrict__ da = xyb_plane(P, s, 1, c, SIDES);
-  float* hbase = P.arena + S.hrow + (long long)(c * kNumQ) * S.plane;
-  const int tid = threadIdx.x;
+// One wave per (scale, channel, quantity, band of kRowsPerWave rows). Lane
+// 3r+k runs cosine term k of row r of the band (cpu.rs:967-1022, row form), so
+// the three terms of a row advance in lock step on neighbouring lanes. Inputs
+// come straight from the XYB planes as float4 (all three lanes of a row read
+// the same address), four columns per group, prefetched four groups ahead
+// into registers; a 16-slot register ring holds the quantity values of
+// columns [n-8, n+8). After each group the three terms are summed in cpu.rs
+// order, (o1 + o3) + o5, through a 1 KB LDS exchange and stored as float4.
+template <int Q>
+__device__ __forceinline__ float row_quantity(float a, float b) {
+  return Q == 0 ? a * a : (Q == 1 ? b * b : (Q == 2 ? a * b : (Q == 3 ? a : b)));
+}
 
-  // Scan lane: row r, quantity slot qi, cosine term k.
-  const bool scan = tid < NQ * 3 * RB;
-  const int r = tid % RB, qk = tid / RB;
-  const int qi = qk / 3, k = qk % 3;
+template <int Q>
+__device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __restrict__ rowb,
+                                         gfloat* __restrict__ out, int w, int pitch, bool ok, int k,
+                                         float4* ks) {
+  constexpr bool NA = Q != 1 && Q != 4;
+  constexpr bool NB = Q == 1 || Q == 2 || Q == 4;
+  const int lane = threadIdx.x;
   const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
   const float mp = k == 0 ? kMulPrev1 : (k == 1 ? kMulPrev3 : kMulPrev5);
-  float prev = 0.0f, prev2 = 0.0f;
-
-  for (int x0 = -kRadius + 1; x0 < w; x0 += CX) {
-    // Phase A: stage columns [x0-6, x0+CX+4) of both inputs, form quantities.
-    for (int e = tid; e < RB * TW; e += kThreads) {
-      const int rr = e / TW, j = e % TW;
-      const int col = x0 - kRadius - 1 + j, row = y0 + rr;
-      float a = 0.0f, b = 0.0f;
-      if (row < h && col >= 0 && col < w) {
-        const long long off = (long long)row * pitch + col;
-        a = ra[off];
-        if (SIDES != kSidesRef) b = da[off];
-      }
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  // Branch-free loads: the address is clamped into the row (pitch >= 64, so
+  // pitch - 4 is a valid float4 start) and invalid columns or rows are zeroed
+  // by a select, so no exec-mask branch splits the prefetch stream.
+  auto load = [&](int col, float4& a, float4& b) {
+    const int cc = min(col, pitch - 4);
+    a = NA ? to_f4(*(gcfloat4*)(rowa + cc)) : z4;
+    b = NB ? to_f4(*(gcfloat4*)(rowb + cc)) : z4;
+  };
+  // Lanes outside the image see width 0, so one compare per value masks both
+  // the row and the column range (no short-circuit branches).
+  const int wl = ok ? w : 0;
+  auto quant = [&](const float4& a, const float4& b, int col, float* v) {
+    v[0] = col + 0 < wl ? row_quantity<Q>(a.x, b.x) : 0.0f;
+    v[1] = col + 1 < wl ? row_quantity<Q>(a.y, b.y) : 0.0f;
+    v[2] = col + 2 < wl ? row_quantity<Q>(a.z, b.z) : 0.0f;
+    v[3] = col + 3 < wl ? row_quantity<Q>(a.w, b.w) : 0.0f;
+  };
+  float R[16];
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) Qs[i][rr][j] = quantity(SideTraits<SIDES>::q(i), a, b);
-    }
-    __syncthreads();
-    // Phase B: CX sequential steps of the row-form recurrence (cpu.rs:967-1022).
-    if (scan) {
-#pragma unroll 8
-      for (int i = 0; i < CX; ++i) {
-        const float sum = Qs[qi][r][i] + Qs[qi][r][i + HALO];
+  for (int i = 0; i < 16; ++i) R[i] = 0.0f;
+  {
+    float4 a, b;
+    load(0, a, b);
+    quant(a, b, 0, R);  // columns 0..3 -> slots 0..3
+  }
+  float4 qa[4], qb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) load(4 + 4 * j, qa[j], qb[j]);
+  float prev = 0.0f, prev2 = 0.0f;
+  // Groups of four outputs cover n = -4 .. w-1; the loop runs whole sets of
+  // four groups so the body is straight-line (groups past the end are masked).
+  const int ngroups = (w + kRadius - 1 + 3) / 4;
+  const int nloop = (ngroups + 3) / 4;
+  for (int g0 = 0; g0 < 4 * nloop; g0 += 4) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n0 = -kRadius + 1 + 4 * (g0 + j);
+      const int base = (12 + 4 * j) & 15;  // n0 mod 16
+      float o4[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float sum = R[(base + i - kRadius - 1 + 16) & 15] + R[(base + i + kRadius - 1) & 15];
         float o = sum * mi;
         o = fmaf(-1.0f, prev2, o);
         prev2 = prev;
         o = fmaf(mp, prev, o);
         prev = o;
-        Os[qk][r][i] = o;
+        o4[i] = o;
       }
-    }
-    __syncthreads();
-    // Phase C: out = (o1 + o3) + o5, coalesced stores of the row-pass planes.
-    for (int e = tid; e < NQ * RB * CX; e += kThreads) {
-      const int i = e % CX, rr = (e / CX) % RB, q2 = e / (CX * RB);
-      const int n = x0 + i, row = y0 + rr;
-      if (n >= 0 && n < w && row < h) {
-        const float v = (Os[q2 * 3][rr][i] + Os[q2 * 3 + 1][rr][i]) + Os[q2 * 3 + 2][rr][i];
-        hbase[(long long)SideTraits<SIDES>::q(q2) * S.plane + (long long)row * pitch + n] = v;
+      // columns n0+8..n0+11 replace n0-8..n0-5 in the ring
+      {
+        float v[4];
+        quant(qa[j], qb[j], n0 + 8, v);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) R[(base + 8 + i) & 15] = v[i];
       }
+      // Keep the refill of slot j below its consumption: if the scheduler
+      // hoisted the load above it, the slot would need two registers and the
+      // loop back-edge copy would wait for the load (vmcnt(0)).
+      __builtin_amdgcn_sched_barrier(0);
+      load(n0 + 24, qa[j], qb[j]);
+      ks[lane] = make_float4(o4[0], o4[1], o4[2], o4[3]);
+      lds_barrier();
+      if (k == 0 && ok && n0 >= 0 && n0 < w) {
+        const float4 x = ks[lane], y = ks[lane + 1], zz = ks[lane + 2];
+        float4 res;
+        res.x = (x.x + y.x) + zz.x;
+        res.y = (x.y + y.y) + zz.y;
+        res.z = (x.z + y.z) + zz.z;
+        res.w = (x.w + y.w) + zz.w;
+        vf4 rv = {res.x, res.y, res.z, res.w};
+        *(gfloat4*)(out + n0) = rv;
+      }
+      lds_barrier();
     }
+  }
+}
+
+template <int SIDES>
+__global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
+                                             const WorkItem* __restrict__ items) {
+  __shared__ float4 ks[64];
+  const WorkItem it = items[blockIdx.x];
+  const PairDesc& P = pairs[it.pair];
+  const int s = it.a, c = it.b / kNumQ, q = it.b % kNumQ;
+  const ScaleDesc& S = P.sc[s];
+  const int lane = threadIdx.x;
+  const int r = lane / 3, k = lane - 3 * r;
+  const int row = it.c * kRowsPerWave + r;
+  const bool ok = r < kRowsPerWave && row < S.h;
+  const long long roff = (long long)(ok ? row : 0) * S.pitch;
+  gcfloat* rowa = gbl(xyb_plane(P, s, 0, c, SIDES) + roff);
+  gcfloat* rowb = gbl(xyb_plane(P, s, 1, c, SIDES) + roff);
+  gfloat* out = gbl(P.arena + S.hrow + (long long)(c * kNumQ + q) * S.plane + roff);
+  switch (q) {
+    case 0: row_scan<0>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
+    case 1: row_scan<1>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
+    case 2: row_scan<2>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
+    case 3: row_scan<3>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
+    default: row_scan<4>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
   }
 }
 
@@ -266,147 +349,177 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// One block per (scale, channel, strip of kColBlock columns). Thread t scans
+// column t%16 for (quantity, cosine term) t/16 (cpu.rs:1054-1115, column form)
+// in chunks of kColChunk rows: the bottom inputs of a chunk are prefetched
+// into registers one chunk ahead, and the top inputs (10 rows up) are the
+// bottom inputs of earlier steps, so each step costs one load. Each chunk's
+// terms go to a double-buffered LDS tile; after one barrier every thread
+// takes one pixel of the chunk, forms (o1+o3)+o5 per quantity and evaluates
+// the SSIM and edge-difference maps with f64 L1/L4 sums (cpu.rs:581-683).
 template <int SIDES>
 __global__ __launch_bounds__(kThreads) void k_cols(const PairDesc* __restrict__ pairs,
                                                    const WorkItem* __restrict__ items) {
   constexpr int NQ = SideTraits<SIDES>::NQ;
-  constexpr int CB = kColBlock, RY = kColChunk, HALO = 2 * kRadius;
-  constexpr int TH = RY + HALO;
-  // Slot strides are 16 mod 32 words so the two 16-lane groups of a half-wave
-  // (different slots, same columns) hit disjoint LDS banks.
-  constexpr int QSTRIDE = TH * CB + 16;
-  constexpr int OSTRIDE = RY * CB + 16;
-  __shared__ float Qs[NQ * QSTRIDE];
-  __shared__ float Os[NQ * 3 * OSTRIDE];
+  constexpr int NC = NQ * 3;
+  constexpr int CB = kColBlock, RY = kColChunk;
+  static_assert(CB * RY == kThreads, "one pixel per thread per chunk");
+  static_assert(RY >= 2 * kRadius, "top inputs come from the previous chunk");
+  __shared__ float O[2][NC][RY][CB];
   __shared__ double red[kThreads / 64][kNumSums];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
-  const int s = it.a, c = it.b, cb = it.c, xb = cb * CB;
+  const int s = it.a, c = it.b, xb = it.c * CB;
   const ScaleDesc& S = P.sc[s];
   const int w = S.w, h = S.h, pitch = S.pitch;
-  const float* hbase = P.arena + S.hrow + (long long)(c * kNumQ) * S.plane;
-  const float* __restrict__ srcp = xyb_plane(P, s, 0, c, SIDES);
-  const float* __restrict__ disp = xyb_plane(P, s, 1, c, SIDES);
-  float* cache_s11 = P.arena + S.cache + (long long)(3 + c) * S.plane;
-  float* cache_mu1 = P.arena + S.cache + (long long)(6 + c) * S.plane;
-  const int tid = threadIdx.x;
-
-  const bool scan = tid < NQ * 3 * CB;
-  const int x = tid % CB, qk = tid / CB;
-  const int qi = qk / 3, k = qk % 3;
+  const int t = threadIdx.x;
+  const int j = t / CB, x = t % CB;
+  const bool scan = j < NC;
+  const int qi = scan ? j / 3 : 0, k = j % 3;
+  const int col = xb + x;
+  const bool colok = col < w;
+  gcfloat* hq = gbl(P.arena + S.hrow + (long long)(c * kNumQ + SideTraits<SIDES>::q(qi)) * S.plane +
+                    min(col, w - 1));
   const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
   const float vmp = k == 0 ? kVertMulPrev1 : (k == 1 ? kVertMulPrev3 : kVertMulPrev5);
+  const bool ld = scan && colok;
+  const unsigned hl = ld ? (unsigned)h : 0u;  // rows outside [0, hl) load as zero
+  // maps-phase pixel of this thread within a chunk
+  const int mi_row = t / CB, mx = t % CB, mcol = xb + mx;
+  gcfloat* srcp = gbl(xyb_plane(P, s, 0, c, SIDES));
+  gcfloat* disp = gbl(xyb_plane(P, s, 1, c, SIDES));
+  gfloat* cache_s11 = gbl(P.arena + S.cache + (long long)(3 + c) * S.plane);
+  gfloat* cache_mu1 = gbl(P.arena + S.cache + (long long)(6 + c) * S.plane);
+
+  // Branch-free: loads from a clamped row, zeroed by a select outside [0, h).
+  auto load_chunk = [&](int y0, float* B) {
+#pragma unroll
+    for (int i = 0; i < RY; ++i) {
+      const int rb = y0 + i + kRadius - 1;
+      const int rc = min(max(rb, 0), h - 1);
+      const float v = hq[(long long)rc * pitch];
+      B[i] = (unsigned)rb < hl ? v : 0.0f;
+    }
+  };
+  float Bprev[RY], Bcur[RY], Bnext[RY];
+#pragma unroll
+  for (int i = 0; i < RY; ++i) Bprev[i] = 0.0f;
+  int y0 = -kRadius + 1;
+  load_chunk(y0, Bcur);
   float prev = 0.0f, prev2 = 0.0f;
   double acc[kNumSums] = {0, 0, 0, 0, 0, 0};
-
-  for (int y0 = -kRadius + 1; y0 < h; y0 += RY) {
-    // Phase A: stage rows [y0-6, y0+RY+4) of the row-pass planes.
-    for (int e = tid; e < NQ * TH * CB; e += kThreads) {
-      const int q2 = e / (TH * CB), rem = e % (TH * CB);
-      const int i = rem / CB, xx = rem % CB;
-      const int row = y0 - kRadius - 1 + i, col = xb + xx;
-      float v = 0.0f;
-      if (row >= 0 && row < h && col < w)
-        v = hbase[(long long)SideTraits<SIDES>::q(q2) * S.plane + (long long)row * pitch + col];
-      Qs[q2 * QSTRIDE + i * CB + xx] = v;
-    }
-    __syncthreads();
-    // Phase B: column-form recurrence (cpu.rs:1054-1115).
+  for (int ci = 0; y0 < h; ++ci, y0 += RY) {
+    load_chunk(y0 + RY, Bnext);
+    const int buf = ci & 1;
     if (scan) {
-      const float* q = Qs + qi * QSTRIDE + x;
-#pragma unroll 8
+#pragma unroll
       for (int i = 0; i < RY; ++i) {
-        const float sum = q[i * CB] + q[(i + HALO) * CB];
-        const float t = fmaf(prev, vmp, prev2);
-        const float o = fmaf(sum, mi, -t);
+        // top input row y0+i-6 = bottom input of step i-10
+        const float top = i >= 2 * kRadius ? Bcur[i - 2 * kRadius] : Bprev[i + RY - 2 * kRadius];
+        const float sum = top + Bcur[i];
+        const float tt = fmaf(prev, vmp, prev2);
+        const float o = fmaf(sum, mi, -tt);
         prev2 = prev;
         prev = o;
-        Os[qk * OSTRIDE + i * CB + x] = o;
+        O[buf][j][i][x] = o;
       }
     }
-    __syncthreads();
-    // Phase C: blurred quantities -> error maps -> f64 sums (or cache writes).
-    for (int e = tid; e < RY * CB; e += kThreads) {
-      const int i = e / CB, xx = e % CB;
-      const int n = y0 + i, col = xb + xx;
-      if (n < 0 || n >= h || col >= w) continue;
+    lds_barrier();
+    const int n = y0 + mi_row;
+    if (n >= 0 && n < h && mcol < w) {
       float bq[NQ];
 #pragma unroll
       for (int q2 = 0; q2 < NQ; ++q2)
-        bq[q2] = (Os[(q2 * 3) * OSTRIDE + e] + Os[(q2 * 3 + 1) * OSTRIDE + e]) + Os[(q2 * 3 + 2) * OSTRIDE + e];
-      const long long off = (long long)n * pitch + col;
+        bq[q2] = (O[buf][3 * q2][mi_row][mx] + O[buf][3 * q2 + 1][mi_row][mx]) + O[buf][3 * q2 + 2][mi_row][mx];
+      const long long off = (long long)n * pitch + mcol;
       if (SIDES == kSidesRef) {
         cache_s11[off] = bq[0];
         cache_mu1[off] = bq[1];
-        continue;
-      }
-      float s11, s22, s12, mu1, mu2;
-      if (SIDES == kSidesBoth) {
-        s11 = bq[0];
-        s22 = bq[1];
-        s12 = bq[2];
-        mu1 = bq[3];
-        mu2 = bq[4];
       } else {
-        s11 = cache_s11[off];
-        mu1 = cache_mu1[off];
-        s22 = bq[0];
-        s12 = bq[1];
-        mu2 = bq[2];
+        float s11, s22, s12, mu1, mu2;
+        if (SIDES == kSidesBoth) {
+          s11 = bq[0];
+          s22 = bq[1];
+          s12 = bq[2];
+          mu1 = bq[3];
+          mu2 = bq[4];
+        } else {
+          s11 = cache_s11[off];
+          mu1 = cache_mu1[off];
+          s22 = bq[0];
+          s12 = bq[1];
+          mu2 = bq[2];
+        }
+        const double d = ssim_error(mu1, mu2, s11, s22, s12);
+        const double d2 = d * d;
+        acc[0] += d;
+        acc[1] += d2 * d2;
+        const double d1 = edge_ratio(srcp[off], mu1, disp[off], mu2);
+        const double art = d1 > 0.0 ? d1 : 0.0;
+        const double det = -d1 > 0.0 ? -d1 : 0.0;
+        const double a2 = art * art, e2 = det * det;
+        acc[2] += art;
+        acc[3] += a2 * a2;
+        acc[4] += det;
+        acc[5] += e2 * e2;
       }
-      const double d = ssim_error(mu1, mu2, s11, s22, s12);
-      const double d2 = d * d;
-      acc[0] += d;
-      acc[1] += d2 * d2;
-      const double d1 = edge_ratio(srcp[off], mu1, disp[off], mu2);
-      const double art = d1 > 0.0 ? d1 : 0.0;
-      const double det = -d1 > 0.0 ? -d1 : 0.0;
-      const double a2 = art * art, e2 = det * det;
-      acc[2] += art;
-      acc[3] += a2 * a2;
-      acc[4] += det;
-      acc[5] += e2 * e2;
+    }
+#pragma unroll
+    for (int i = 0; i < RY; ++i) {
+      Bprev[i] = Bcur[i];
+      Bcur[i] = Bnext[i];
     }
   }
   if (SIDES == kSidesRef) return;
-  // Block reduction in a fixed order: wave shuffles, then wave 0..3 in sequence.
-  const int lane = tid & 63, wv = tid >> 6;
+  // Block reduction in a fixed order: wave shuffles, then waves 0..3 in sequence.
+  const int lane = t & 63, wv = t >> 6;
 #pragma unroll
-  for (int j = 0; j < kNumSums; ++j) {
-    const double v = wave_sum(acc[j]);
-    if (lane == 0) red[wv][j] = v;
+  for (int q = 0; q < kNumSums; ++q) {
+    const double v = wave_sum(acc[q]);
+    if (lane == 0) red[wv][q] = v;
   }
-  __syncthreads();
-  if (tid < kNumSums) {
+  lds_barrier();
+  if (t < kNumSums) {
     double v = 0.0;
-    for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
-    P.partials[(long long)(S.part + c * S.colblocks + cb) * kNumSums + tid] = v;
+    for (int q = 0; q < kThreads / 64; ++q) v += red[q][t];
+    P.partials[(long long)(S.part + c * S.colblocks + it.c) * kNumSums + t] = v;
   }
 }
 
 // ---------------------------------------------------------------- finalize
 
-// cpu.rs:633-634, 676-679 (averages) and cpu.rs:840-868 (weighted sum, transform).
-__global__ __launch_bounds__(128) void k_finalize(const PairDesc* __restrict__ pairs) {
+// One block of 1024 threads per pair. Wave w reduces the partial records of
+// (scale, channel) groups w and w+16 in a fixed order (strided lane sums, then
+// a shuffle tree), so the result does not depend on timing. Thread 0 then
+// forms the 108 features and the score (cpu.rs:633-634, 676-679, 840-868).
+__global__ __launch_bounds__(1024) void k_finalize(const PairDesc* __restrict__ pairs) {
   const PairDesc& P = pairs[blockIdx.x];
   __shared__ double sums[kSumsPerPair];
-  const int t = threadIdx.x;
-  if (t < kSumsPerPair) {
-    const int s = t / 18, c = (t / kNumSums) % 3, j = t % kNumSums;
-    double v = 0.0;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int g = wv; g < kMaxScales * 3; g += 16) {
+    const int s = g / 3, c = g % 3;
+    double v[kNumSums] = {0, 0, 0, 0, 0, 0};
     if (s < P.nscales) {
       const ScaleDesc& S = P.sc[s];
-      const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums + j;
-      for (int b = 0; b < S.colblocks; ++b) v += p[(long long)b * kNumSums];
+      const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums;
+      for (int b = lane; b < S.colblocks; b += 64)
+#pragma unroll
+        for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
     }
-    sums[t] = v;
-    P.result[1 + t] = v;
+#pragma unroll
+    for (int q = 0; q < kNumSums; ++q) {
+      const double r = wave_sum(v[q]);
+      if (lane == 0) {
+        sums[g * kNumSums + q] = r;
+        P.result[1 + g * kNumSums + q] = r;
+      }
+    }
   }
   if (t < 2 * kMaxScales) {
     const int s = t / 2;
     P.result[1 + kSumsPerPair + t] = s < P.nscales ? (double)(t & 1 ? P.sc[s].h : P.sc[s].w) : 0.0;
   }
-  __syncthreads();
+  lds_barrier();
   if (t != 0) return;
   double ssim = 0.0;
   int i = 0;
@@ -470,7 +583,7 @@ hipError_t launch_pyramid(const PairDesc* pairs, const WorkItem* items, int nite
 hipError_t launch_rows(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
                        hipStream_t stream) {
   if (nitems <= 0) return hipSuccess;
-  SSIM2_DISPATCH_SIDES(k_rows, dim3(nitems), dim3(kThreads), stream, pairs, items);
+  SSIM2_DISPATCH_SIDES(k_rows, dim3(nitems), dim3(64), stream, pairs, items);
   return hipGetLastError();
 }
 
@@ -483,7 +596,7 @@ hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems,
 
 hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream) {
   if (npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3(npairs), dim3(128), 0, stream, pairs);
+  hipLaunchKernelGGL(k_finalize, dim3(npairs), dim3(1024), 0, stream, pairs);
   return hipGetLastError();
 }
 

@@ -41,8 +41,9 @@ struct PairPlan {
   PairDesc desc{};               // arena/partials/result pointers left null
   long long arena_elems = 0;     // floats
   int nparts = 0;                // partial-sum records
-  std::vector<WorkItem> items;   // pyramid items, then rows, then cols
-  int off_pyr = 0, n_pyr = 0, off_rows = 0, n_rows = 0, off_cols = 0, n_cols = 0;
+  std::vector<WorkItem> items;   // pyramid items, row items per sides, column items
+  int off_pyr = 0, n_pyr = 0, off_cols = 0, n_cols = 0;
+  int off_rows[3] = {0, 0, 0}, n_rows[3] = {0, 0, 0};  // indexed by Sides
 };
 
 // Lay out one pair's planes starting at element `arena_base` and its partial
@@ -78,12 +79,18 @@ inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPla
   for (int ty = 0; ty < (h + kPyrTile - 1) / kPyrTile; ++ty)
     for (int tx = 0; tx < (w + kPyrTile - 1) / kPyrTile; ++tx) p.items.push_back({pair, tx, ty, 0});
   p.n_pyr = (int)p.items.size();
-  // Row pass: (scale, channel, 16-row band), largest scale first.
-  p.off_rows = (int)p.items.size();
-  for (int s = 0; s < d.nscales; ++s)
-    for (int c = 0; c < 3; ++c)
-      for (int b = 0; b < (sh[s] + kRowBlock - 1) / kRowBlock; ++b) p.items.push_back({pair, s, c, b});
-  p.n_rows = (int)p.items.size() - p.off_rows;
+  // Row pass: one wave per (scale, channel*5+quantity, band of kRowsPerWave
+  // rows), largest scale first; one list per Sides (its quantities only).
+  static const int kQ[3][5] = {{0, 1, 2, 3, 4}, {0, 3, -1, -1, -1}, {1, 2, 4, -1, -1}};
+  for (int sd = 0; sd < 3; ++sd) {
+    p.off_rows[sd] = (int)p.items.size();
+    for (int s = 0; s < d.nscales; ++s)
+      for (int c = 0; c < 3; ++c)
+        for (int qi = 0; qi < 5 && kQ[sd][qi] >= 0; ++qi)
+          for (int b = 0; b < (sh[s] + kRowsPerWave - 1) / kRowsPerWave; ++b)
+            p.items.push_back({pair, s, c * kNumQ + kQ[sd][qi], b});
+    p.n_rows[sd] = (int)p.items.size() - p.off_rows[sd];
+  }
   // Column pass + maps: (scale, channel, 16-column strip).
   p.off_cols = (int)p.items.size();
   for (int s = 0; s < d.nscales; ++s)

@@ -1,0 +1,304 @@
+// Serving runtime for the fast form (include/ssim2hip.h, ssim2_server_*).
+//
+// Requests (pairs of device-resident sRGB images) are submitted from any
+// thread. A scheduler thread keeps `slots` batch engines, each on its own
+// stream: whenever a slot is idle it takes the oldest waiting requests that
+// fit the slot's capacity (pairs, pixels) and the HBM budget, and launches
+// them; requests that arrive while other batches run join the next batch, so
+// their work overlaps on the device (continuous batching). Completion is
+// detected per slot with hipEventQuery and time-stamped on the host clock.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ssim2hip.h"
+#include "ssim2_internal.h"
+#include "ssim2_plan.h"
+
+using namespace ssim2;
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+ssim2_status sfail(ssim2_status st, const std::string& m) {
+  set_last_error(m);
+  return st;
+}
+
+// Device bytes a request occupies while in flight in the fast form: XYB of
+// scales >= 1 in the storage type plus its partial-sum records.
+uint64_t request_bytes(uint32_t w, uint32_t h, uint32_t storage) {
+  int sw[kMaxScales], sh[kMaxScales];
+  const int n = scale_dims((int)w, (int)h, sw, sh);
+  const uint64_t sb = storage == SSIM2_STORAGE_F32 ? 4 : 2;
+  uint64_t b = 0;
+  for (int s = 0; s < n; ++s) {
+    const uint64_t pitch = (uint64_t(sw[s]) + 15) / 16 * 16;
+    if (s > 0) b += 6 * pitch * sh[s] * sb + 256;
+    b += 3 * (uint64_t(sw[s] + 15) / 16) * ((sh[s] + 15) / 16) * kNumSums * sizeof(double);
+  }
+  return b + kResultDoubles * sizeof(double);
+}
+
+struct Request {
+  ssim2_pair pair;
+  uint64_t id;
+  double t_submit;
+  uint64_t px;
+  uint64_t bytes;
+};
+
+struct Slot {
+  ssim2_batch* batch = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<Request> reqs;
+  std::vector<ssim2_pair> pairs;
+  std::vector<double> scores;
+  uint64_t bytes = 0;
+  bool busy = false;
+  double t_launch = 0;
+};
+
+}  // namespace
+
+struct ssim2_server {
+  ssim2_server_options opt{};
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Request> waiting;
+  std::vector<ssim2_result> done;
+  uint64_t in_flight_bytes = 0;
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  int device = 0;
+  // statistics
+  uint64_t n_submitted = 0, n_done = 0, n_refused = 0, n_batches = 0, px_done = 0;
+  double busy_since = 0, first_submit = 0, last_done = 0;
+  std::string worker_error;
+
+  void run();
+};
+
+void ssim2_server::run() {
+  (void)hipSetDevice(device);
+  while (!stop.load()) {
+    bool progressed = false;
+    // 1. retire finished batches
+    for (Slot& s : slots) {
+      if (!s.busy) continue;
+      int fin = 0;
+      if (ssim2_batch_query(s.batch, &fin) != SSIM2_OK) fin = 1;
+      if (!fin) continue;
+      const double t = now_s();
+      s.scores.resize(s.reqs.size());
+      const ssim2_status st = ssim2_batch_get_scores(s.batch, s.scores.data(), (uint32_t)s.scores.size());
+      std::lock_guard<std::mutex> lk(mu);
+      for (size_t i = 0; i < s.reqs.size(); ++i) {
+        ssim2_result r{};
+        r.request_id = s.reqs[i].id;
+        r.score = st == SSIM2_OK ? s.scores[i] : 0.0;
+        r.status = st;
+        r.submit_time_s = s.reqs[i].t_submit;
+        r.done_time_s = t;
+        done.push_back(r);
+        px_done += s.reqs[i].px;
+      }
+      n_done += s.reqs.size();
+      last_done = t;
+      in_flight_bytes -= s.bytes;
+      s.busy = false;
+      s.reqs.clear();
+      progressed = true;
+      cv.notify_all();
+    }
+    // 2. launch waiting requests on idle slots
+    for (Slot& s : slots) {
+      if (s.busy) continue;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        uint64_t px = 0, bytes = 0;
+        while (!waiting.empty() && s.reqs.size() < opt.max_pairs_per_batch) {
+          const Request& q = waiting.front();
+          // FIFO admission: stop at the first request that does not fit the
+          // slot's pixel capacity or the HBM budget left by the batches in
+          // flight (it waits for the next slot).
+          if (px + q.px > opt.max_pixels_per_batch) break;
+          if (in_flight_bytes + bytes + q.bytes > opt.hbm_budget) break;
+          px += q.px;
+          bytes += q.bytes;
+          s.reqs.push_back(q);
+          waiting.pop_front();
+        }
+        if (s.reqs.empty()) continue;
+        s.bytes = bytes;
+        in_flight_bytes += bytes;
+      }
+      s.pairs.resize(s.reqs.size());
+      for (size_t i = 0; i < s.reqs.size(); ++i) s.pairs[i] = s.reqs[i].pair;
+      s.t_launch = now_s();
+      const ssim2_status st = ssim2_batch_compute_async(s.batch, s.pairs.data(), (uint32_t)s.pairs.size(), s.stream);
+      if (st != SSIM2_OK) {
+        std::lock_guard<std::mutex> lk(mu);
+        worker_error = ssim2_last_error();
+        for (const Request& q : s.reqs) {
+          ssim2_result r{};
+          r.request_id = q.id;
+          r.status = st;
+          r.submit_time_s = q.t_submit;
+          r.done_time_s = now_s();
+          done.push_back(r);
+        }
+        n_done += s.reqs.size();
+        in_flight_bytes -= s.bytes;
+        s.reqs.clear();
+        cv.notify_all();
+        continue;
+      }
+      s.busy = true;
+      ++n_batches;
+      progressed = true;
+    }
+    if (!progressed) {
+      std::unique_lock<std::mutex> lk(mu);
+      bool any_busy = false;
+      for (const Slot& s : slots) any_busy |= s.busy;
+      if (any_busy)
+        cv.wait_for(lk, std::chrono::microseconds(20));
+      else
+        cv.wait_for(lk, std::chrono::milliseconds(2), [&] { return stop.load() || !waiting.empty(); });
+    }
+  }
+}
+
+extern "C" {
+
+ssim2_status ssim2_server_create(const ssim2_server_options* o, ssim2_server** out) {
+  if (!o || !out) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (o->struct_size != sizeof(ssim2_server_options))
+    return sfail(SSIM2_INVALID_ARGUMENT, "ssim2_server_options.struct_size mismatch");
+  if (o->slots == 0 || o->slots > 16 || o->max_pairs_per_batch == 0 || o->max_pixels_per_batch < 64 ||
+      o->hbm_budget == 0)
+    return sfail(SSIM2_INVALID_ARGUMENT, "slots in [1,16], non-zero batch limits and HBM budget required");
+  ssim2_server* sv = new (std::nothrow) ssim2_server();
+  if (!sv) return sfail(SSIM2_OOM, "host allocation failed");
+  sv->opt = *o;
+  (void)hipGetDevice(&sv->device);
+  sv->slots.resize(o->slots);
+  for (Slot& s : sv->slots) {
+    ssim2_batch_options bo{};
+    bo.struct_size = sizeof bo;
+    bo.storage = o->storage;
+    bo.max_pairs = o->max_pairs_per_batch;
+    bo.max_pixels = o->max_pixels_per_batch;
+    bo.memory_budget = 0;  // engine capacity is static; hbm_budget governs admission
+    ssim2_status st = ssim2_batch_create(&bo, &s.batch);
+    hipError_t e = hipSuccess;
+    if (st == SSIM2_OK) e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+    if (st != SSIM2_OK || e != hipSuccess) {
+      const std::string m = st != SSIM2_OK ? std::string(ssim2_last_error()) : hipGetErrorString(e);
+      for (Slot& t : sv->slots) {
+        if (t.batch) ssim2_batch_destroy(t.batch);
+        if (t.stream) (void)hipStreamDestroy(t.stream);
+      }
+      delete sv;
+      return sfail(st != SSIM2_OK ? st : SSIM2_HIP_ERROR, "server slot: " + m);
+    }
+    if (o->tile_cfg || o->grid_mult) ssim2_batch_set_tiling(s.batch, o->tile_cfg, o->grid_mult ? o->grid_mult : 4);
+  }
+  sv->worker = std::thread([sv] { sv->run(); });
+  *out = sv;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_server_submit(ssim2_server* sv, const ssim2_pair* pair, uint64_t request_id) {
+  if (!sv || !pair || !pair->ref.ptr || !pair->dis.ptr) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (pair->ref.width != pair->dis.width || pair->ref.height != pair->dis.height)
+    return sfail(SSIM2_DIM_MISMATCH, "reference and distorted sizes differ");
+  if (!size_supported(pair->ref.width, pair->ref.height)) return sfail(SSIM2_INVALID_SIZE, "unsupported size");
+  Request q;
+  q.pair = *pair;
+  q.id = request_id;
+  q.px = uint64_t(pair->ref.width) * pair->ref.height;
+  q.bytes = request_bytes(pair->ref.width, pair->ref.height, sv->opt.storage);
+  if (q.bytes > sv->opt.hbm_budget || q.px > sv->opt.max_pixels_per_batch) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    ++sv->n_refused;
+    return sfail(SSIM2_ADMISSION_REFUSED, "request needs " + std::to_string(q.bytes) + " B / " +
+                                              std::to_string(q.px) + " px, above the server limits");
+  }
+  q.t_submit = now_s();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->n_submitted == 0) sv->first_submit = q.t_submit;
+    ++sv->n_submitted;
+    sv->waiting.push_back(q);
+  }
+  sv->cv.notify_all();
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_server_poll(ssim2_server* sv, ssim2_result* out, uint32_t capacity, uint32_t* n) {
+  if (!sv || !n || (!out && capacity)) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  const uint32_t k = (uint32_t)std::min<size_t>(capacity, sv->done.size());
+  for (uint32_t i = 0; i < k; ++i) out[i] = sv->done[i];
+  sv->done.erase(sv->done.begin(), sv->done.begin() + k);
+  *n = k;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_server_drain(ssim2_server* sv, double timeout_s) {
+  if (!sv) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  const bool ok = sv->cv.wait_for(lk, std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 1e9),
+                                  [&] { return sv->n_done >= sv->n_submitted; });
+  return ok ? SSIM2_OK : sfail(SSIM2_NOT_READY, "drain timed out");
+}
+
+ssim2_status ssim2_server_stats(ssim2_server* sv, ssim2_server_stats_t* st) {
+  if (!sv || !st) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  std::memset(st, 0, sizeof *st);
+  st->struct_size = sizeof *st;
+  st->submitted = sv->n_submitted;
+  st->completed = sv->n_done;
+  st->refused = sv->n_refused;
+  st->batches = sv->n_batches;
+  st->pixels_completed = sv->px_done;
+  st->waiting = sv->waiting.size();
+  st->in_flight_bytes = sv->in_flight_bytes;
+  st->first_submit_s = sv->first_submit;
+  st->last_done_s = sv->last_done;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_server_destroy(ssim2_server* sv) {
+  if (!sv) return SSIM2_OK;
+  sv->stop.store(true);
+  sv->cv.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  for (Slot& s : sv->slots) {
+    if (s.batch) ssim2_batch_destroy(s.batch);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+  }
+  delete sv;
+  return SSIM2_OK;
+}
+
+}  // extern "C"

@@ -29,7 +29,13 @@ __all__ = [
     "lib",
     "LIB_PATH",
     "STATUS",
+    "Ssim2Batch",
+    "Ssim2Server",
+    "STORAGE",
 ]
+
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+TILE_SHAPES = [(32, 16), (64, 16), (32, 32), (16, 16)]
 
 STATUS = {
     0: "SSIM2_OK",
@@ -60,6 +66,61 @@ class _Image(ctypes.Structure):
         ("pitch_bytes", ctypes.c_size_t),
         ("width", ctypes.c_uint32),
         ("height", ctypes.c_uint32),
+    ]
+
+
+class _Pair(ctypes.Structure):
+    _fields_ = [("ref", _Image), ("dis", _Image)]
+
+
+class _BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("storage", ctypes.c_uint32),
+        ("max_pairs", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("max_pixels", ctypes.c_uint64),
+        ("memory_budget", ctypes.c_uint64),
+    ]
+
+
+class _ServerOptions(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("storage", ctypes.c_uint32),
+        ("slots", ctypes.c_uint32),
+        ("max_pairs_per_batch", ctypes.c_uint32),
+        ("max_pixels_per_batch", ctypes.c_uint64),
+        ("hbm_budget", ctypes.c_uint64),
+        ("tile_cfg", ctypes.c_uint32),
+        ("grid_mult", ctypes.c_uint32),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("request_id", ctypes.c_uint64),
+        ("score", ctypes.c_double),
+        ("submit_time_s", ctypes.c_double),
+        ("done_time_s", ctypes.c_double),
+        ("status", ctypes.c_int),
+        ("reserved0", ctypes.c_uint32),
+    ]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("refused", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("pixels_completed", ctypes.c_uint64),
+        ("waiting", ctypes.c_uint64),
+        ("in_flight_bytes", ctypes.c_uint64),
+        ("first_submit_s", ctypes.c_double),
+        ("last_done_s", ctypes.c_double),
     ]
 
 
@@ -111,6 +172,23 @@ def _load():
         "ssim2_device_free": (vp,),
         "ssim2_device_copy": (vp, vp, sz, vp),
         "ssim2_device_synchronize": (),
+        "ssim2_batch_create": (P(_BatchOptions), P(vp)),
+        "ssim2_batch_destroy": (vp,),
+        "ssim2_batch_mem_usage": (vp, P(sz)),
+        "ssim2_batch_compute_async": (vp, P(_Pair), u32, vp),
+        "ssim2_batch_get_scores": (vp, P(dbl), u32),
+        "ssim2_batch_compute_sync": (vp, P(_Pair), u32, vp, P(dbl)),
+        "ssim2_batch_query": (vp, P(ctypes.c_int)),
+        "ssim2_batch_get_sums": (vp, u32, P(dbl), P(ctypes.c_int)),
+        "ssim2_batch_set_tiling": (vp, u32, u32),
+        "ssim2_batch_get_tiling": (vp, P(u32), P(u32)),
+        "ssim2_batch_autotune": (vp, P(_Pair), u32, vp, u32, ctypes.c_char_p, sz),
+        "ssim2_server_create": (P(_ServerOptions), P(vp)),
+        "ssim2_server_destroy": (vp,),
+        "ssim2_server_submit": (vp, P(_Pair), u64),
+        "ssim2_server_poll": (vp, P(_Result), u32, P(u32)),
+        "ssim2_server_drain": (vp, dbl),
+        "ssim2_server_stats": (vp, P(_ServerStats)),
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -399,3 +477,154 @@ class Ssimulacra2:
                 raise Ssim2Error(8, "no images bound: pass ref_linear/dis_linear or use Ssimulacra2.new")
             return self._bound
         return ref_linear, dis_linear
+
+
+class Ssim2Batch:
+    """Batched fast form: many sRGB pairs of any sizes per call (include/ssim2hip.h ssim2_batch_*).
+
+    Scores follow the FP64 evaluation of the metric (exact FIR blur, shifted
+    statistics); they are not bit-identical to `Ssimulacra2`'s exact path.
+    """
+
+    def __init__(self, max_pairs: int, max_pixels: int, storage: str = "f32", memory_budget: int = 0):
+        o = _BatchOptions(ctypes.sizeof(_BatchOptions), STORAGE[storage], max_pairs, 0, max_pixels, memory_budget)
+        h = ctypes.c_void_p()
+        _check(lib.ssim2_batch_create(ctypes.byref(o), ctypes.byref(h)))
+        self._b = h
+        self.storage = storage
+        self.max_pairs = max_pairs
+        self._last = None
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib.ssim2_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def _pairs(pairs):
+        arr = (_Pair * len(pairs))()
+        for i, (r, d) in enumerate(pairs):
+            arr[i].ref = r._img
+            arr[i].dis = d._img
+        return arr
+
+    def compute_async(self, pairs, stream=None):
+        """pairs: sequence of (ref DeviceImage u8, dis DeviceImage u8)."""
+        arr = self._pairs(pairs)
+        self._last = (arr, len(pairs))
+        _check(lib.ssim2_batch_compute_async(self._b, arr, len(pairs), _stream_ptr(stream)))
+
+    def get_scores(self):
+        n = self._last[1]
+        out = (ctypes.c_double * n)()
+        _check(lib.ssim2_batch_get_scores(self._b, out, n))
+        return np.array(out[:], dtype=np.float64)
+
+    def compute(self, pairs, stream=None):
+        self.compute_async(pairs, stream)
+        return self.get_scores()
+
+    def done(self) -> bool:
+        v = ctypes.c_int()
+        _check(lib.ssim2_batch_query(self._b, ctypes.byref(v)))
+        return bool(v.value)
+
+    def sums(self, index: int):
+        s = (ctypes.c_double * 108)()
+        d = (ctypes.c_int * 12)()
+        _check(lib.ssim2_batch_get_sums(self._b, index, s, d))
+        return np.array(s[:]).reshape(6, 3, 6), np.array(d[:], dtype=np.int32).reshape(6, 2)
+
+    def set_tiling(self, tile_cfg: int, grid_mult: int):
+        _check(lib.ssim2_batch_set_tiling(self._b, tile_cfg, grid_mult))
+
+    def tiling(self):
+        a, b = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib.ssim2_batch_get_tiling(self._b, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, pairs, reps: int = 3, stream=None) -> str:
+        arr = self._pairs(pairs)
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib.ssim2_batch_autotune(self._b, arr, len(pairs), _stream_ptr(stream), reps, buf, 4096))
+        return buf.value.decode()
+
+    def mem_usage(self) -> int:
+        v = ctypes.c_size_t()
+        _check(lib.ssim2_batch_mem_usage(self._b, ctypes.byref(v)))
+        return v.value
+
+
+class Ssim2Server:
+    """Continuous-batching scorer (include/ssim2hip.h ssim2_server_*).
+
+    submit() is thread-safe and returns at once; results come back through
+    poll() with host timestamps. Submitted images must stay alive until their
+    result has been polled (this wrapper keeps references until then).
+    """
+
+    def __init__(self, storage="f16", slots=3, max_pairs_per_batch=256, max_pixels_per_batch=64 << 20,
+                 hbm_budget=64 << 30, tile_cfg=0, grid_mult=0):
+        o = _ServerOptions(ctypes.sizeof(_ServerOptions), STORAGE[storage], slots, max_pairs_per_batch,
+                           max_pixels_per_batch, hbm_budget, tile_cfg, grid_mult)
+        h = ctypes.c_void_p()
+        _check(lib.ssim2_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self._s = h
+        self._keep = {}
+        self._buf = (_Result * 4096)()
+
+    def submit(self, ref: DeviceImage, dis: DeviceImage, request_id: int):
+        p = _Pair(ref._img, dis._img)
+        _check(lib.ssim2_server_submit(self._s, ctypes.byref(p), request_id))
+        self._keep[request_id] = (ref, dis)
+
+    def poll(self):
+        """Finished results as a list of (id, score, t_submit, t_done, status)."""
+        out = []
+        while True:
+            n = ctypes.c_uint32()
+            _check(lib.ssim2_server_poll(self._s, self._buf, len(self._buf), ctypes.byref(n)))
+            for i in range(n.value):
+                r = self._buf[i]
+                out.append((r.request_id, r.score, r.submit_time_s, r.done_time_s, STATUS.get(r.status, r.status)))
+                self._keep.pop(r.request_id, None)
+            if n.value < len(self._buf):
+                return out
+
+    def drain(self, timeout_s: float = 0.0):
+        _check(lib.ssim2_server_drain(self._s, timeout_s))
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(lib.ssim2_server_stats(self._s, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_ if not k.startswith(("struct", "reserved"))}
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib.ssim2_server_destroy(self._s)
+            self._s = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
