@@ -78,16 +78,15 @@ struct Codec<_Float16> {
   __device__ static _Float16 enc(float v) { return (_Float16)v; }
   __device__ static float dec(_Float16 v) { return (float)v; }
 };
-struct bf16_t {
-  unsigned short bits;
-};
+// bfloat16 is carried as its 16-bit pattern.
+typedef unsigned short bf16_t;
 template <>
 struct Codec<bf16_t> {
   __device__ static bf16_t enc(float v) {  // round to nearest even (finite inputs)
     const unsigned u = __float_as_uint(v);
-    return bf16_t{(unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16)};
+    return (bf16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
   }
-  __device__ static float dec(bf16_t v) { return __uint_as_float((unsigned)v.bits << 16); }
+  __device__ static float dec(bf16_t v) { return __uint_as_float((unsigned)v << 16); }
 };
 
 __device__ __forceinline__ void fast_barrier() {
@@ -96,12 +95,15 @@ __device__ __forceinline__ void fast_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Global address space (see ssim2_kernels.hip: generic accesses force full
+// waits on every outstanding load).
 template <typename T>
-__device__ __forceinline__ T* fast_plane(const PairDesc& P, int s, int img, int c) {
+__device__ __forceinline__ __attribute__((address_space(1))) T* fast_plane(const PairDesc& P, int s, int img, int c) {
   const ScaleDesc& S = P.sc[s];
-  return reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(P.arena) + S.xyb) +
-         (long long)(img * 3 + c) * S.plane;
+  return (__attribute__((address_space(1))) T*)(reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(P.arena) + S.xyb) +
+                                               (long long)(img * 3 + c) * S.plane);
 }
+typedef __attribute__((address_space(1))) const unsigned char gcu8;
 
 // Sum of the in-range taps of the 1-D FIR at position g of a line of n (the
 // 1-D blur of ones); exactly 1 away from the border.
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
       if (gx >= P.w || gy >= P.h) continue;
 #pragma unroll
       for (int img = 0; img < 2; ++img) {
-        const unsigned char* px = (img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
+        gcu8* px = (gcu8*)(img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
 #pragma unroll
         for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = c_lut_fast[px[c]];
       }
@@ -242,8 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
         if (s == 0) {
 #pragma unroll
           for (int img = 0; img < 2; ++img) {
-            const unsigned char* px =
-                (img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
+            gcu8* px = (gcu8*)(img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
             float X, Y, B;
             xyb_fast(c_lut_fast[px[0]], c_lut_fast[px[1]], c_lut_fast[px[2]], X, Y, B);
             v[img][0] = X - xyb_shift(0);
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
       double v = 0.0;
       for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
       const int c = tid / 6, j = tid % 6;
-      P.partials[(long long)(S.part + c * S.colblocks + tile) * kNumSums + j] = v;
+      ((__attribute__((address_space(1))) double*)P.partials)[(long long)(S.part + c * S.colblocks + tile) * kNumSums + j] = v;
     }
   }
 }

@@ -68,6 +68,8 @@ typedef __attribute__((address_space(1))) const vf4 gcfloat4;
 __device__ __forceinline__ float4 to_f4(vf4 v) { return make_float4(v.x, v.y, v.z, v.w); }
 __device__ __forceinline__ gcfloat* gbl(const float* p) { return (gcfloat*)p; }
 __device__ __forceinline__ gfloat* gbl(float* p) { return (gfloat*)p; }
+typedef __attribute__((address_space(1))) const unsigned char gcuchar;
+__device__ __forceinline__ gcuchar* gbl(const unsigned char* p) { return (gcuchar*)p; }
 
 // Workgroup barrier that orders LDS only. __syncthreads() also waits for every
 // outstanding global load and store (vmcnt(0)), which would drain the
@@ -133,24 +135,24 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
     if (gx >= w || gy >= h) continue;
     for (int im = 0; im < NI; ++im) {
       const int img = SIDES == kSidesDis ? 1 : im;
-      const unsigned char* base = img == 0 ? in.ref : in.dis;
+      gcuchar* base = gbl(img == 0 ? in.ref : in.dis);
       const long long pitch = img == 0 ? in.ref_pitch : in.dis_pitch;
       float rgb[3];
       if (in.kind == kInputSrgbU8) {
-        const unsigned char* px = base + gy * pitch + gx * 3;
+        gcuchar* px = base + gy * pitch + gx * 3;
         rgb[0] = c_lut[px[0]];
         rgb[1] = c_lut[px[1]];
         rgb[2] = c_lut[px[2]];
         float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
         if (lo) {
           const long long lp = img == 0 ? in.lin_ref_pitch : in.lin_dis_pitch;
-          float* q = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(lo) + gy * lp) + gx * 3;
+          gfloat* q = gbl(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(lo) + gy * lp)) + gx * 3;
           q[0] = rgb[0];
           q[1] = rgb[1];
           q[2] = rgb[2];
         }
       } else {
-        const float* px = reinterpret_cast<const float*>(base + gy * pitch) + gx * 3;
+        gcfloat* px = (gcfloat*)(base + gy * pitch) + gx * 3;
         rgb[0] = px[0];
         rgb[1] = px[1];
         rgb[2] = px[2];
@@ -158,9 +160,9 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
       float X, Y, B;
       linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
       const long long off = (long long)gy * P.sc[0].pitch + gx;
-      xyb_plane(P, 0, img, 0, SIDES)[off] = X;
-      xyb_plane(P, 0, img, 1, SIDES)[off] = Y;
-      xyb_plane(P, 0, img, 2, SIDES)[off] = B;
+      gbl(xyb_plane(P, 0, img, 0, SIDES))[off] = X;
+      gbl(xyb_plane(P, 0, img, 1, SIDES))[off] = Y;
+      gbl(xyb_plane(P, 0, img, 2, SIDES))[off] = B;
       for (int c = 0; c < 3; ++c) bufA[im * 3 + c][ly][lx] = rgb[c];
     }
   }
@@ -206,9 +208,9 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
         float X, Y, B;
         linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
         const long long off = (long long)gy * P.sc[s].pitch + gx;
-        xyb_plane(P, s, img, 0, SIDES)[off] = X;
-        xyb_plane(P, s, img, 1, SIDES)[off] = Y;
-        xyb_plane(P, s, img, 2, SIDES)[off] = B;
+        gbl(xyb_plane(P, s, img, 0, SIDES))[off] = X;
+        gbl(xyb_plane(P, s, img, 1, SIDES))[off] = Y;
+        gbl(xyb_plane(P, s, img, 2, SIDES))[off] = B;
       }
     }
   }
@@ -270,13 +272,21 @@ __device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __
   float prev = 0.0f, prev2 = 0.0f;
   // Groups of four outputs cover n = -4 .. w-1; the loop runs whole sets of
   // four groups so the body is straight-line (groups past the end are masked).
+  // The three-term sum of group g-1 is read back from LDS while group g's
+  // recurrence runs (software pipelining): a wave's LDS operations execute in
+  // issue order, so only compiler reordering has to be prevented.
   const int ngroups = (w + kRadius - 1 + 3) / 4;
-  const int nloop = (ngroups + 3) / 4;
+  const int nloop = (ngroups + 3) / 4 + 1;  // one extra set drains the last sum
+  const int l1 = (lane + 1) & 63, l2 = (lane + 2) & 63;
   for (int g0 = 0; g0 < 4 * nloop; g0 += 4) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int n0 = -kRadius + 1 + 4 * (g0 + j);
       const int base = (12 + 4 * j) & 15;  // n0 mod 16
+      // 1. issue the reads of the previous group's terms
+      float4* kp = ks + ((j + 1) & 1) * 64;  // buffer written by group g-1
+      const float4 x = kp[lane], y = kp[l1], zz = kp[l2];
+      // 2. recurrence of this group
       float o4[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -288,6 +298,16 @@ __device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __
         prev = o;
         o4[i] = o;
       }
+      // 3. (o1 + o3) + o5 of the previous group, stored by the k = 0 lane
+      const int np = n0 - 4;
+      if (k == 0 && ok && np >= 0 && np < w) {
+        vf4 rv = {(x.x + y.x) + zz.x, (x.y + y.y) + zz.y, (x.z + y.z) + zz.z, (x.w + y.w) + zz.w};
+        *(gfloat4*)(out + np) = rv;
+      }
+      // 4. publish this group's terms
+      asm volatile("" ::: "memory");
+      ks[(j & 1) * 64 + lane] = make_float4(o4[0], o4[1], o4[2], o4[3]);
+      asm volatile("" ::: "memory");
       // columns n0+8..n0+11 replace n0-8..n0-5 in the ring
       {
         float v[4];
@@ -300,19 +320,6 @@ __device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __
       // loop back-edge copy would wait for the load (vmcnt(0)).
       __builtin_amdgcn_sched_barrier(0);
       load(n0 + 24, qa[j], qb[j]);
-      ks[lane] = make_float4(o4[0], o4[1], o4[2], o4[3]);
-      lds_barrier();
-      if (k == 0 && ok && n0 >= 0 && n0 < w) {
-        const float4 x = ks[lane], y = ks[lane + 1], zz = ks[lane + 2];
-        float4 res;
-        res.x = (x.x + y.x) + zz.x;
-        res.y = (x.y + y.y) + zz.y;
-        res.z = (x.z + y.z) + zz.z;
-        res.w = (x.w + y.w) + zz.w;
-        vf4 rv = {res.x, res.y, res.z, res.w};
-        *(gfloat4*)(out + n0) = rv;
-      }
-      lds_barrier();
     }
   }
 }
@@ -320,7 +327,7 @@ __device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __
 template <int SIDES>
 __global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
                                              const WorkItem* __restrict__ items) {
-  __shared__ float4 ks[64];
+  __shared__ float4 ks[2 * 64];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b / kNumQ, q = it.b % kNumQ;
