@@ -51,6 +51,26 @@ typedef struct {
 
 typedef struct ssim2_ctx ssim2_ctx;
 
+/* YUV 4:2:0 frame in device memory, BT.709 limited range (the video input of
+ * crates/turbo-metrics/src/color.rs:96-116). SSIM2_YUV420P10: three planes of
+ * u16 10-bit samples (LSB-aligned), c0 = Cb, c1 = Cr, chroma planes
+ * ceil(w/2) x ceil(h/2). SSIM2_P016: two planes of u16 samples in the high
+ * bits (the NVDEC P016 layout the reference consumes), c0 = interleaved CbCr,
+ * c1 unused. */
+typedef enum { SSIM2_YUV420P10 = 0, SSIM2_P016 = 1 } ssim2_yuv_format;
+
+typedef struct {
+  const void* y;
+  const void* c0;
+  const void* c1;
+  size_t y_pitch_bytes;
+  size_t c_pitch_bytes;
+  uint32_t width;
+  uint32_t height;
+  uint32_t format; /* ssim2_yuv_format */
+  uint32_t reserved0;
+} ssim2_yuv_image;
+
 /* Context options. Zero-initialise, then set struct_size = sizeof(ssim2_options). */
 typedef struct {
   uint32_t struct_size;
@@ -112,6 +132,15 @@ ssim2_status ssim2_compute_srgb_async(ssim2_ctx* ctx, const ssim2_image* ref_srg
  * context's own staging images. */
 ssim2_status ssim2_compute_from_host_srgb(ssim2_ctx* ctx, const uint8_t* ref, const uint8_t* dis,
                                           size_t row_bytes, ssim2_stream stream, double* score);
+
+/* Video frames: YUV 4:2:0 converted to linear RGB inside the first kernel
+ * (fused ingest), then the same exact pipeline. The reference converts with a
+ * separate kernel into the buffers its graph is bound to
+ * (crates/turbo-metrics/src/lib.rs:271-363, color.rs:96-116). */
+ssim2_status ssim2_compute_yuv420_async(ssim2_ctx* ctx, const ssim2_yuv_image* ref,
+                                        const ssim2_yuv_image* dis, ssim2_stream stream);
+ssim2_status ssim2_compute_yuv420_sync(ssim2_ctx* ctx, const ssim2_yuv_image* ref,
+                                       const ssim2_yuv_image* dis, ssim2_stream stream, double* score);
 
 /* ---- cached reference ---------------------------------------------------- */
 
@@ -182,6 +211,13 @@ ssim2_status ssim2_batch_mem_usage(const ssim2_batch* batch, size_t* bytes);
 /* Enqueue one batch on `stream` (waits first for this object's previous batch). */
 ssim2_status ssim2_batch_compute_async(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
                                        ssim2_stream stream);
+/* As ssim2_batch_compute_async for YUV 4:2:0 frame pairs (fused ingest). */
+typedef struct {
+  ssim2_yuv_image ref;
+  ssim2_yuv_image dis;
+} ssim2_yuv_pair;
+ssim2_status ssim2_batch_compute_yuv_async(ssim2_batch* batch, const ssim2_yuv_pair* pairs, uint32_t npairs,
+                                           ssim2_stream stream);
 /* Scores of the last batch, in submission order (waits for it). */
 ssim2_status ssim2_batch_get_scores(ssim2_batch* batch, double* scores, uint32_t capacity);
 ssim2_status ssim2_batch_compute_sync(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
@@ -281,6 +317,12 @@ ssim2_status ssim2_device_synchronize(void);
 ssim2_status ssim2_srgb_to_linear(const ssim2_image* src_u8, ssim2_image* dst_f32,
                                   ssim2_stream stream);
 
+/* CudaColorspace::biplanaryuv420_to_linearrgb_16 (crates/cuda-colorspace/src/
+ * lib.rs:79-123; kernel crates/cuda-colorspace-kernel/src/biplanar.rs:8-70):
+ * YUV 4:2:0 BT.709 limited -> packed f32 linear RGB (the power function is
+ * the device powf, where the reference uses its fast approximation). */
+ssim2_status ssim2_yuv420_to_linear(const ssim2_yuv_image* src, ssim2_image* dst_f32, ssim2_stream stream);
+
 /* Device image helpers, the C form of cudarse-npp Image::malloc /
  * copy_from_cpu / copy_to_cpu (crates/cudarse/cudarse-npp/src/image/isu.rs:18-40,
  * image/mod.rs:410-439). bytes_per_sample is 1 (u8) or 4 (f32). Upload and
@@ -290,6 +332,16 @@ ssim2_status ssim2_image_alloc(uint32_t width, uint32_t height, uint32_t bytes_p
 ssim2_status ssim2_image_free(ssim2_image* image);
 ssim2_status ssim2_image_upload(ssim2_image* image, const void* host, size_t host_row_bytes);
 ssim2_status ssim2_image_download(const ssim2_image* image, void* host, size_t host_row_bytes);
+
+/* YUV 4:2:0 frames in device memory: the decoded surfaces the reference's
+ * video loop receives from NVDEC (crates/cudarse/cudarse-video/src/dec.rs,
+ * consumed at crates/turbo-metrics/src/lib.rs:271-363). Planes get 256-byte
+ * aligned pitches. Upload takes tightly packed host planes: Y w*h u16; for
+ * SSIM2_YUV420P10 Cb and Cr ceil(w/2)*ceil(h/2) u16; for SSIM2_P016 one
+ * interleaved CbCr plane 2*ceil(w/2)*ceil(h/2) u16 (c1 ignored). */
+ssim2_status ssim2_yuv_alloc(uint32_t width, uint32_t height, uint32_t format, ssim2_yuv_image* out);
+ssim2_status ssim2_yuv_free(ssim2_yuv_image* image);
+ssim2_status ssim2_yuv_upload(ssim2_yuv_image* image, const uint16_t* y, const uint16_t* c0, const uint16_t* c1);
 
 /* Stream helpers (cudarse-driver CuStream::new / sync, stream.rs:46,107). */
 ssim2_status ssim2_stream_create(ssim2_stream* out);

@@ -45,6 +45,11 @@
 //   * whole pipeline: identical inputs score exactly 100 (cpu.rs:862-868),
 //     JPEG corpus scores of the published ssimulacra2 0.5.1 crate (loose),
 //     monotone in distortion strength, FP32 mirror vs FP64 within bounds.
+//   * YUV 4:2:0 -> linear (video input): the reference holds no vectors for
+//     it; pinned by the BT.709 Kr/Kb of Rec. ITU-R BT.709 (0.2126, 0.0722),
+//     neutral chroma giving grey, the luma floor/ceiling giving 0/1, the
+//     inverse OETF's continuity at the threshold, and a forward-encode ->
+//     decode round trip (tests/test_oracle_yuv.py).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -528,9 +533,98 @@ int pipeline(const T* ref_lin, const T* dis_lin, int w0, int h0, int mode, doubl
   return nscales;
 }
 
+// ---- YUV 4:2:0 BT.709 limited range -> linear RGB, in double ----
+// crates/cuda-colorspace-kernel/src/biplanar.rs:8-70: 16-bit-range samples,
+// luma floor 16<<8 over a range of (235-16)<<8, chroma neutral 1<<15 over
+// (240-16)<<8, one chroma sample per 2x2 quad; lib.rs:185-201 derives the
+// matrix from Kr/Kb, lib.rs:205-223 (constants_from_primaries) gets Kr/Kb from
+// the BT.709 primaries and D65 white, lib.rs:206-223 the inverse OETF.
+struct YuvCoeffs {
+  double y, r, b, g1, g2;
+};
+
+YuvCoeffs bt709_coeffs() {
+  struct V3 {
+    double x, y, z;
+  };
+  auto xyz = [](double x, double y) { return V3{x / y, 1.0, (1.0 - x - y) / y}; };
+  auto cross = [](V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+  auto dot = [](V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+  const V3 r = xyz(0.640, 0.330), g = xyz(0.300, 0.600), b = xyz(0.150, 0.060), w = xyz(0.3127, 0.3290);
+  const V3 xr{r.x, g.x, b.x}, yr{r.y, g.y, b.y}, zr{r.z, g.z, b.z};
+  const double mul = 1.0 / dot(xr, cross(yr, zr));
+  const double kr = dot(w, cross(g, b)) * mul, kb = dot(w, cross(r, g)) * mul;
+  const double kg = 1.0 - kr - kb;
+  const double luma_range = double((235 - 16) << 8), chroma_range = double((240 - 16) << 8);
+  return {1.0 / luma_range, 2.0 * (1.0 - kr) / chroma_range, 2.0 * (1.0 - kb) / chroma_range,
+          -2.0 * (1.0 - kb) * kb / kg / chroma_range, -2.0 * (1.0 - kr) * kr / kg / chroma_range};
+}
+
+double bt709_eotf_f64(double v) {
+  const double beta = 0.018053968510807, alpha = 1.0 + 5.5 * beta;
+  const double threshold = alpha * std::pow(beta, 0.45) - (alpha - 1.0);  // OETF(beta)
+  return v >= threshold ? std::pow((v + (alpha - 1.0)) / alpha, 1.0 / 0.45) : v / 4.5;
+}
+
 }  // namespace
 
 extern "C" {
+
+double oracle_score_from_sums(const double* sums, const int* dims, int nscales);
+
+// Kr, Kb and the five matrix coefficients the oracle derives (for the pins).
+void oracle_bt709_coeffs(double* out7) {
+  const YuvCoeffs c = bt709_coeffs();
+  const double m[7] = {c.y, c.r, c.b, c.g1, c.g2, 0, 0};
+  std::memcpy(out7, m, sizeof(m));
+  // Kr and Kb back from r and b: r = 2(1-Kr)/range
+  const double chroma_range = double((240 - 16) << 8);
+  out7[5] = 1.0 - c.r * chroma_range / 2.0;
+  out7[6] = 1.0 - c.b * chroma_range / 2.0;
+}
+
+double oracle_bt709_eotf(double v) { return bt709_eotf_f64(v); }
+
+// Planar 10-bit (LSB-aligned u16) 4:2:0 frame -> packed linear RGB double,
+// w*h*3. Chroma planes are ceil(w/2) x ceil(h/2), packed.
+void oracle_yuv420p10_to_linear_f64(const uint16_t* y, const uint16_t* cb, const uint16_t* cr, int w, int h,
+                                    double* out) {
+  const YuvCoeffs c = bt709_coeffs();
+  const int cw = (w + 1) / 2;
+  for (int py = 0; py < h; ++py)
+    for (int px = 0; px < w; ++px) {
+      const size_t ci = size_t(py / 2) * cw + px / 2;
+      const double u = double(int(cb[ci] << 6) - 32768), v = double(int(cr[ci] << 6) - 32768);
+      const int yy = int(y[size_t(py) * w + px]) << 6;
+      const double luma = double(std::max(yy, 4096) - 4096) * c.y;
+      const double rgb[3] = {luma + c.r * v, luma + c.g1 * u + c.g2 * v, luma + c.b * u};
+      for (int k = 0; k < 3; ++k)
+        out[(size_t(py) * w + px) * 3 + k] = std::min(std::max(bt709_eotf_f64(rgb[k]), 0.0), 1.0);
+    }
+}
+
+// FP64 score of a pair of packed linear RGB double images (MODE_F64 pipeline).
+int oracle_score_linear_f64(const double* ref, const double* dis, int w, int h, double* score) {
+  if (w <= 0 || h <= 0) return -1;
+  double sums[108] = {0};
+  int dims[12] = {0};
+  const int ns = pipeline<double>(ref, dis, w, h, 1, sums, dims);
+  *score = oracle_score_from_sums(sums, dims, ns);
+  return ns;
+}
+
+// FP64 score of a 10-bit 4:2:0 frame pair (conversion and pipeline in double).
+int oracle_score_yuv420p10_f64(const uint16_t* yr, const uint16_t* ur, const uint16_t* vr, const uint16_t* yd,
+                               const uint16_t* ud, const uint16_t* vd, int w, int h, double* score) {
+  std::vector<double> a(size_t(w) * h * 3), b(size_t(w) * h * 3);
+  oracle_yuv420p10_to_linear_f64(yr, ur, vr, w, h, a.data());
+  oracle_yuv420p10_to_linear_f64(yd, ud, vd, w, h, b.data());
+  double sums[108] = {0};
+  int dims[12] = {0};
+  const int ns = pipeline<double>(a.data(), b.data(), w, h, 1, sums, dims);
+  *score = oracle_score_from_sums(sums, dims, ns);
+  return ns;
+}
 
 // ---- per-stage entry points (for stage parity tests) ----
 

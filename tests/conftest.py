@@ -46,6 +46,12 @@ class Oracle:
             getattr(L, n).argtypes = [vp, i, i, vp]
         L.oracle_map_sums_f32.argtypes = [i] + [vp] * 7 + [vp]
         L.oracle_score_srgb_many.argtypes = [vp, vp, vp, vp, i, i, i, vp]
+        L.oracle_bt709_coeffs.argtypes = [vp]
+        L.oracle_bt709_eotf.argtypes = [d]
+        L.oracle_bt709_eotf.restype = d
+        L.oracle_yuv420p10_to_linear_f64.argtypes = [vp, vp, vp, i, i, vp]
+        L.oracle_score_linear_f64.argtypes = [vp, vp, i, i, ctypes.POINTER(d)]
+        L.oracle_score_yuv420p10_f64.argtypes = [vp] * 6 + [i, i, ctypes.POINTER(d)]
         self.L = L
 
     def score_srgb(self, ref, dis, mode=MIRROR, with_sums=False):
@@ -109,6 +115,38 @@ class Oracle:
         out = np.zeros(((h + 1) // 2, (w + 1) // 2, 3), np.float32)
         self.L.oracle_downscale_f32(a.ctypes.data, w, h, out.ctypes.data)
         return out
+
+    def bt709(self):
+        """(y, r, b, g1, g2, Kr, Kb) of the double derivation."""
+        c = np.zeros(7)
+        self.L.oracle_bt709_coeffs(c.ctypes.data)
+        return c
+
+    def yuv_to_linear(self, y, cb, cr):
+        """10-bit planes -> (h, w, 3) float64 linear RGB."""
+        y = np.ascontiguousarray(y, np.uint16)
+        cb = np.ascontiguousarray(cb, np.uint16)
+        cr = np.ascontiguousarray(cr, np.uint16)
+        h, w = y.shape
+        out = np.empty((h, w, 3))
+        self.L.oracle_yuv420p10_to_linear_f64(y.ctypes.data, cb.ctypes.data, cr.ctypes.data, w, h, out.ctypes.data)
+        return out
+
+    def score_linear_f64(self, ref, dis):
+        a = np.ascontiguousarray(ref, np.float64)
+        b = np.ascontiguousarray(dis, np.float64)
+        h, w = a.shape[:2]
+        s = ctypes.c_double()
+        self.L.oracle_score_linear_f64(a.ctypes.data, b.ctypes.data, w, h, ctypes.byref(s))
+        return s.value
+
+    def score_yuv_f64(self, ref, dis):
+        """ref, dis: (Y, Cb, Cr) 10-bit planes."""
+        planes = [np.ascontiguousarray(p, np.uint16) for p in (*ref, *dis)]
+        h, w = planes[0].shape
+        s = ctypes.c_double()
+        self.L.oracle_score_yuv420p10_f64(*[p.ctypes.data for p in planes], w, h, ctypes.byref(s))
+        return s.value
 
     def scale_dims(self, w, h):
         d = np.zeros(12, np.int32)

@@ -88,6 +88,41 @@ ssim2_status check_image(const ssim2_ctx* c, const ssim2_image* im, size_t bps, 
   return SSIM2_OK;
 }
 
+ssim2_status check_yuv(const ssim2_yuv_image* im, uint32_t w, uint32_t h, const char* name) {
+  if (!im || !im->y || !im->c0 || (im->format == SSIM2_YUV420P10 && !im->c1))
+    return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + ": null plane");
+  if (im->format > SSIM2_P016) return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + ": unknown YUV format");
+  if (im->width != w || im->height != h)
+    return fail(SSIM2_DIM_MISMATCH, std::string(name) + " is " + std::to_string(im->width) + "x" +
+                                        std::to_string(im->height) + ", expected " + std::to_string(w) + "x" +
+                                        std::to_string(h));
+  const size_t cw = (size_t(im->width) + 1) / 2;
+  if (im->y_pitch_bytes < 2 * size_t(im->width) ||
+      im->c_pitch_bytes < (im->format == SSIM2_P016 ? 4 : 2) * cw || (im->y_pitch_bytes & 1) || (im->c_pitch_bytes & 1))
+    return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + ": pitch too small or odd");
+  return SSIM2_OK;
+}
+
+void yuv_inputs(InputArgs* in, const ssim2_yuv_image* ref, const ssim2_yuv_image* dis) {
+  if (ref) {
+    in->ref = static_cast<const unsigned char*>(ref->y);
+    in->ref_c0 = static_cast<const unsigned char*>(ref->c0);
+    in->ref_c1 = static_cast<const unsigned char*>(ref->c1);
+    in->ref_pitch = (long long)ref->y_pitch_bytes;
+    in->ref_cpitch = (long long)ref->c_pitch_bytes;
+  }
+  if (dis) {
+    in->dis = static_cast<const unsigned char*>(dis->y);
+    in->dis_c0 = static_cast<const unsigned char*>(dis->c0);
+    in->dis_c1 = static_cast<const unsigned char*>(dis->c1);
+    in->dis_pitch = (long long)dis->y_pitch_bytes;
+    in->dis_cpitch = (long long)dis->c_pitch_bytes;
+  }
+  const ssim2_yuv_image* a = ref ? ref : dis;
+  in->kind = a->format == SSIM2_P016 ? kInputYuvP016 : kInputYuvP10;
+  in->use = 1;
+}
+
 InputArgs make_inputs(const ssim2_image* ref, const ssim2_image* dis, int kind) {
   InputArgs in{};
   in.ref = ref ? static_cast<const unsigned char*>(ref->ptr) : nullptr;
@@ -302,6 +337,26 @@ ssim2_status ssim2_compute_from_host_srgb(ssim2_ctx* c, const uint8_t* ref, cons
   return ssim2_get_score(c, score);
 }
 
+ssim2_status ssim2_compute_yuv420_async(ssim2_ctx* c, const ssim2_yuv_image* ref, const ssim2_yuv_image* dis,
+                                        ssim2_stream stream) {
+  if (!c) return fail(SSIM2_INVALID_ARGUMENT, "ctx is null");
+  ssim2_status st;
+  if ((st = check_yuv(ref, c->w, c->h, "ref")) != SSIM2_OK) return st;
+  if ((st = check_yuv(dis, c->w, c->h, "dis")) != SSIM2_OK) return st;
+  if (ref->format != dis->format) return fail(SSIM2_INVALID_ARGUMENT, "ref and dis YUV formats differ");
+  InputArgs in{};
+  yuv_inputs(&in, ref, dis);
+  return enqueue(c, kSidesBoth, in, (hipStream_t)stream);
+}
+
+ssim2_status ssim2_compute_yuv420_sync(ssim2_ctx* c, const ssim2_yuv_image* ref, const ssim2_yuv_image* dis,
+                                       ssim2_stream stream, double* score) {
+  if (!score) return fail(SSIM2_INVALID_ARGUMENT, "score is null");
+  ssim2_status st = ssim2_compute_yuv420_async(c, ref, dis, stream);
+  if (st != SSIM2_OK) return st;
+  return ssim2_get_score(c, score);
+}
+
 // ---- cached reference
 
 static ssim2_status set_reference(ssim2_ctx* c, const ssim2_image* ref, int kind, hipStream_t s) {
@@ -463,6 +518,20 @@ ssim2_status ssim2_srgb_to_linear(const ssim2_image* src, ssim2_image* dst, ssim
   return SSIM2_OK;
 }
 
+ssim2_status ssim2_yuv420_to_linear(const ssim2_yuv_image* src, ssim2_image* dst, ssim2_stream stream) {
+  if (!src || !dst || !dst->ptr) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st = check_yuv(src, dst->width, dst->height, "src");
+  if (st != SSIM2_OK) return st;
+  if (dst->pitch_bytes < size_t(dst->width) * 12 || dst->pitch_bytes % 4)
+    return fail(SSIM2_INVALID_ARGUMENT, "dst pitch too small or misaligned");
+  InputArgs in{};
+  yuv_inputs(&in, src, nullptr);
+  InputView v{in.ref, in.ref_c0, in.ref_c1, in.ref_pitch, in.ref_cpitch, in.kind};
+  HIP_TRY(launch_yuv_to_linear(v, static_cast<float*>(dst->ptr), (long long)dst->pitch_bytes, (int)dst->width,
+                               (int)dst->height, (hipStream_t)stream));
+  return SSIM2_OK;
+}
+
 ssim2_status ssim2_image_alloc(uint32_t width, uint32_t height, uint32_t bps, ssim2_image* out) {
   if (!out || (bps != 1 && bps != 4) || width == 0 || height == 0)
     return fail(SSIM2_INVALID_ARGUMENT, "bad image_alloc arguments");
@@ -503,6 +572,48 @@ ssim2_status ssim2_image_download(const ssim2_image* im, void* host, size_t host
   const size_t row = std::min(host_row_bytes, im->pitch_bytes);
   HIP_TRY(hipMemcpy2D(host, host_row_bytes, im->ptr, im->pitch_bytes, row, im->height,
                       hipMemcpyDeviceToHost));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_yuv_alloc(uint32_t w, uint32_t h, uint32_t format, ssim2_yuv_image* out) {
+  if (!out) return fail(SSIM2_INVALID_ARGUMENT, "out is null");
+  *out = ssim2_yuv_image{};
+  if (w == 0 || h == 0 || w > kMaxDim || h > kMaxDim) return fail(SSIM2_INVALID_SIZE, "bad frame size");
+  if (format > SSIM2_P016) return fail(SSIM2_INVALID_ARGUMENT, "unknown YUV format");
+  const size_t cw = (size_t(w) + 1) / 2, ch = (size_t(h) + 1) / 2;
+  const size_t yp = (size_t(w) * 2 + 255) & ~size_t(255);
+  const size_t cp = ((format == SSIM2_P016 ? 4 : 2) * cw + 255) & ~size_t(255);
+  const size_t nplanes_c = format == SSIM2_P016 ? 1 : 2;
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, yp * h + nplanes_c * cp * ch));
+  unsigned char* b = static_cast<unsigned char*>(p);
+  out->y = b;
+  out->c0 = b + yp * h;
+  out->c1 = format == SSIM2_P016 ? nullptr : b + yp * h + cp * ch;
+  out->y_pitch_bytes = yp;
+  out->c_pitch_bytes = cp;
+  out->width = w;
+  out->height = h;
+  out->format = format;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_yuv_free(ssim2_yuv_image* im) {
+  if (!im) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (im->y) HIP_TRY(hipFree(const_cast<void*>(im->y)));
+  *im = ssim2_yuv_image{};
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_yuv_upload(ssim2_yuv_image* im, const uint16_t* y, const uint16_t* c0, const uint16_t* c1) {
+  if (!im || !im->y || !y || !c0 || (im->format == SSIM2_YUV420P10 && !c1))
+    return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  const size_t w = im->width, h = im->height, cw = (w + 1) / 2, ch = (h + 1) / 2;
+  HIP_TRY(hipMemcpy2D(const_cast<void*>(im->y), im->y_pitch_bytes, y, w * 2, w * 2, h, hipMemcpyHostToDevice));
+  const size_t crow = (im->format == SSIM2_P016 ? 4 : 2) * cw;
+  HIP_TRY(hipMemcpy2D(const_cast<void*>(im->c0), im->c_pitch_bytes, c0, crow, crow, ch, hipMemcpyHostToDevice));
+  if (im->format == SSIM2_YUV420P10)
+    HIP_TRY(hipMemcpy2D(const_cast<void*>(im->c1), im->c_pitch_bytes, c1, crow, crow, ch, hipMemcpyHostToDevice));
   return SSIM2_OK;
 }
 

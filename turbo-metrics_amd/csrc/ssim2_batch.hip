@@ -132,7 +132,7 @@ ssim2_status enqueue_kernels(ssim2_batch* b, hipStream_t s, int cfg, int mult) {
   const int* meta = meta_ptr(b->d_index);
   BHIP(launch_fast_pyramid(b->storage, b->d_desc, pyr_prefix(b, b->d_index), meta, b->cus * 2, s));
   BHIP(launch_fast_tile(b->storage, cfg, b->d_desc, tile_prefix(b, b->d_index), meta, b->cus * mult, s));
-  BHIP(launch_fast_finalize(b->d_desc, meta, std::min<int>(b->cus, (int)b->max_pairs), s));
+  BHIP(launch_fast_finalize(b->d_desc, meta, b->cus * 4, (int)b->max_pairs, s));
   return SSIM2_OK;
 }
 
@@ -165,7 +165,8 @@ ssim2_status ensure_graph(ssim2_batch* b) {
   return SSIM2_OK;
 }
 
-ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg) {
+ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
+                   const ssim2_yuv_pair* ypairs = nullptr) {
   if (n == 0) return bfail(SSIM2_BATCH_SIZE, "empty batch");
   if (n > b->max_pairs)
     return bfail(SSIM2_BATCH_SIZE, std::to_string(n) + " pairs exceed the batch capacity of " +
@@ -178,23 +179,57 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg)
   int* pp = pyr_prefix(b, b->h_index);
   int* tp = tile_prefix(b, b->h_index);
   for (uint32_t i = 0; i < n; ++i) {
-    const ssim2_pair& q = pairs[i];
-    if (!q.ref.ptr || !q.dis.ptr) return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": null image");
-    if (q.ref.width != q.dis.width || q.ref.height != q.dis.height)
-      return bfail(SSIM2_DIM_MISMATCH, "pair " + std::to_string(i) + ": reference and distorted sizes differ");
-    if (q.ref.pitch_bytes < size_t(q.ref.width) * 3 || q.dis.pitch_bytes < size_t(q.dis.width) * 3)
-      return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": pitch smaller than a row");
+    uint32_t qw, qh;
+    if (ypairs) {
+      const ssim2_yuv_pair& y = ypairs[i];
+      qw = y.ref.width;
+      qh = y.ref.height;
+      if (y.dis.width != qw || y.dis.height != qh)
+        return bfail(SSIM2_DIM_MISMATCH, "pair " + std::to_string(i) + ": reference and distorted sizes differ");
+      if (y.ref.format != y.dis.format || y.ref.format > SSIM2_P016 || !y.ref.y || !y.dis.y || !y.ref.c0 ||
+          !y.dis.c0 || (y.ref.format == SSIM2_YUV420P10 && (!y.ref.c1 || !y.dis.c1)) ||
+          y.ref.y_pitch_bytes < 2 * size_t(qw) || y.dis.y_pitch_bytes < 2 * size_t(qw) ||
+          y.ref.c_pitch_bytes < (y.ref.format == SSIM2_P016 ? 4 : 2) * ((size_t(qw) + 1) / 2) ||
+          y.dis.c_pitch_bytes < (y.dis.format == SSIM2_P016 ? 4 : 2) * ((size_t(qw) + 1) / 2) ||
+          ((y.ref.y_pitch_bytes | y.dis.y_pitch_bytes | y.ref.c_pitch_bytes | y.dis.c_pitch_bytes) & 1))
+        return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": bad YUV planes");
+    } else {
+      const ssim2_pair& q = pairs[i];
+      qw = q.ref.width;
+      qh = q.ref.height;
+      if (!q.ref.ptr || !q.dis.ptr) return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": null image");
+      if (q.ref.width != q.dis.width || q.ref.height != q.dis.height)
+        return bfail(SSIM2_DIM_MISMATCH, "pair " + std::to_string(i) + ": reference and distorted sizes differ");
+      if (q.ref.pitch_bytes < size_t(q.ref.width) * 3 || q.dis.pitch_bytes < size_t(q.dis.width) * 3)
+        return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": pitch smaller than a row");
+    }
     FastPlan fp;
-    if (!plan_fast((int)q.ref.width, (int)q.ref.height, b->storage, cfg, arena, parts, &fp))
-      return bfail(SSIM2_INVALID_SIZE, "pair " + std::to_string(i) + ": size " + std::to_string(q.ref.width) +
-                                           "x" + std::to_string(q.ref.height) + " unsupported");
-    px += uint64_t(q.ref.width) * q.ref.height;
+    if (!plan_fast((int)qw, (int)qh, b->storage, cfg, arena, parts, &fp))
+      return bfail(SSIM2_INVALID_SIZE, "pair " + std::to_string(i) + ": size " + std::to_string(qw) + "x" +
+                                           std::to_string(qh) + " unsupported");
+    px += uint64_t(qw) * qh;
     PairDesc d = fp.d;
-    d.in_ref = static_cast<const unsigned char*>(q.ref.ptr);
-    d.in_dis = static_cast<const unsigned char*>(q.dis.ptr);
-    d.in_ref_pitch = (long long)q.ref.pitch_bytes;
-    d.in_dis_pitch = (long long)q.dis.pitch_bytes;
-    d.in_kind = kInputSrgbU8;
+    if (ypairs) {
+      const ssim2_yuv_pair& y = ypairs[i];
+      d.in_ref = static_cast<const unsigned char*>(y.ref.y);
+      d.in_dis = static_cast<const unsigned char*>(y.dis.y);
+      d.in_ref_pitch = (long long)y.ref.y_pitch_bytes;
+      d.in_dis_pitch = (long long)y.dis.y_pitch_bytes;
+      d.in_ref_c0 = static_cast<const unsigned char*>(y.ref.c0);
+      d.in_ref_c1 = static_cast<const unsigned char*>(y.ref.c1);
+      d.in_dis_c0 = static_cast<const unsigned char*>(y.dis.c0);
+      d.in_dis_c1 = static_cast<const unsigned char*>(y.dis.c1);
+      d.in_ref_cpitch = (long long)y.ref.c_pitch_bytes;
+      d.in_dis_cpitch = (long long)y.dis.c_pitch_bytes;
+      d.in_kind = y.ref.format == SSIM2_P016 ? kInputYuvP016 : kInputYuvP10;
+    } else {
+      const ssim2_pair& q = pairs[i];
+      d.in_ref = static_cast<const unsigned char*>(q.ref.ptr);
+      d.in_dis = static_cast<const unsigned char*>(q.dis.ptr);
+      d.in_ref_pitch = (long long)q.ref.pitch_bytes;
+      d.in_dis_pitch = (long long)q.dis.pitch_bytes;
+      d.in_kind = kInputSrgbU8;
+    }
     d.arena = reinterpret_cast<float*>(b->arena);
     d.partials = b->partials;
     d.result = b->results + size_t(i) * kResultDoubles;
@@ -321,6 +356,14 @@ ssim2_status ssim2_batch_compute_async(ssim2_batch* b, const ssim2_pair* pairs, 
                                        ssim2_stream stream) {
   if (!b || (!pairs && npairs)) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
   ssim2_status st = stage(b, pairs, npairs, b->tile_cfg);
+  if (st != SSIM2_OK) return st;
+  return submit(b, npairs, (hipStream_t)stream, true, b->tile_cfg, b->grid_mult);
+}
+
+ssim2_status ssim2_batch_compute_yuv_async(ssim2_batch* b, const ssim2_yuv_pair* pairs, uint32_t npairs,
+                                           ssim2_stream stream) {
+  if (!b || (!pairs && npairs)) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st = stage(b, nullptr, npairs, b->tile_cfg, pairs);
   if (st != SSIM2_OK) return st;
   return submit(b, npairs, (hipStream_t)stream, true, b->tile_cfg, b->grid_mult);
 }

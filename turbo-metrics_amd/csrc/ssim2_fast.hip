@@ -25,7 +25,8 @@
 //   k_fast_tile     one TXxTY output tile of one scale of one pair per item:
 //                   XYB tile + 4 px halo -> products -> horizontal FIR ->
 //                   vertical FIR -> maps -> f64 block sums.
-//   k_fast_finalize one pair per item: fixed-order reduction and score.
+//   finalisation    k_reduce_groups + k_score (ssim2_finalize.hip).
+#include "ssim2_input.h"
 #include "ssim2_internal.h"
 #include "ssim2_math.h"
 #include "ssim2_tables.h"
@@ -35,7 +36,6 @@ namespace ssim2 {
 namespace {
 
 __constant__ float c_lut_fast[256] = SSIM2_SRGB8_LUT_INIT;
-__constant__ double c_weight_fast[108] = SSIM2_WEIGHTS_INIT;
 
 // 9-tap FIR (centre first), the double taps of the recursive Gaussian rounded to f32.
 constexpr float kH0 = 0x1.0ef8d6p-2f, kH1 = 0x1.b413e8p-3f, kH2 = 0x1.bfd672p-4f,
@@ -164,9 +164,10 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
       if (gx >= P.w || gy >= P.h) continue;
 #pragma unroll
       for (int img = 0; img < 2; ++img) {
-        gcu8* px = (gcu8*)(img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
+        float rgb[3];
+        read_linear(input_of(P, img), gx, gy, c_lut_fast, rgb);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = c_lut_fast[px[c]];
+        for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = rgb[c];
       }
     }
     for (int s = 1; s < P.nscales; ++s) {
@@ -244,9 +245,10 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
         if (s == 0) {
 #pragma unroll
           for (int img = 0; img < 2; ++img) {
-            gcu8* px = (gcu8*)(img ? P.in_dis : P.in_ref) + gy * (img ? P.in_dis_pitch : P.in_ref_pitch) + gx * 3;
+            float rgb[3];
+            read_linear(input_of(P, img), gx, gy, c_lut_fast, rgb);
             float X, Y, B;
-            xyb_fast(c_lut_fast[px[0]], c_lut_fast[px[1]], c_lut_fast[px[2]], X, Y, B);
+            xyb_fast(rgb[0], rgb[1], rgb[2], X, Y, B);
             v[img][0] = X - xyb_shift(0);
             v[img][1] = Y - xyb_shift(1);
             v[img][2] = B - xyb_shift(2);
@@ -369,61 +371,6 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
   }
 }
 
-// ---------------------------------------------------------------- finalize
-
-__global__ __launch_bounds__(kThreads) void k_fast_finalize(const PairDesc* __restrict__ pairs,
-                                                            const int* __restrict__ meta) {
-  __shared__ double sums[kSumsPerPair];
-  const int n = meta[0];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int pi = blockIdx.x; pi < n; pi += gridDim.x) {
-    const PairDesc& P = pairs[pi];
-    fast_barrier();
-    for (int g = wv; g < kMaxScales * 3; g += kThreads / 64) {
-      const int s = g / 3, c = g % 3;
-      double v[kNumSums] = {0, 0, 0, 0, 0, 0};
-      if (s < P.nscales) {
-        const ScaleDesc& S = P.sc[s];
-        const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums;
-        for (int b = lane; b < S.colblocks; b += 64)
-#pragma unroll
-          for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
-      }
-#pragma unroll
-      for (int q = 0; q < kNumSums; ++q) {
-        const double r = wave_sum_d(v[q]);
-        if (lane == 0) {
-          sums[g * kNumSums + q] = r;
-          P.result[1 + g * kNumSums + q] = r;
-        }
-      }
-    }
-    if (t < 2 * kMaxScales) {
-      const int s = t / 2;
-      P.result[1 + kSumsPerPair + t] = s < P.nscales ? (double)(t & 1 ? P.sc[s].h : P.sc[s].w) : 0.0;
-    }
-    fast_barrier();
-    if (t == 0) {
-      double ssim = 0.0;
-      int i = 0;
-      for (int c = 0; c < 3; ++c)
-        for (int s = 0; s < P.nscales; ++s) {
-          const double opp = 1.0 / (double)((long long)P.sc[s].w * P.sc[s].h);
-          const double* q = sums + s * 18 + c * kNumSums;
-          const double l1[3] = {opp * q[0], opp * q[2], opp * q[4]};
-          const double l4[3] = {sqrt(sqrt(opp * q[1])), sqrt(sqrt(opp * q[3])), sqrt(sqrt(opp * q[5]))};
-          for (int m = 0; m < 3; ++m) ssim = fma(c_weight_fast[i++], fabs(l1[m]), ssim);
-          for (int m = 0; m < 3; ++m) ssim = fma(c_weight_fast[i++], fabs(l4[m]), ssim);
-        }
-      ssim *= 0.9562382616834844;
-      ssim = fma(6.248496625763138e-5 * ssim * ssim, ssim,
-                 fma(2.326765642916932, ssim, -0.020884521182843837 * ssim * ssim));
-      ssim = ssim > 0.0 ? fma(pow(ssim, 0.6276336467831387), -10.0, 100.0) : 100.0;
-      P.result[0] = ssim;
-    }
-  }
-}
-
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta,
@@ -471,9 +418,9 @@ hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, co
   }
 }
 
-hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(k_fast_finalize, dim3(grid), dim3(kThreads), 0, stream, pairs, meta);
-  return hipGetLastError();
+hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, int max_pairs,
+                                hipStream_t stream) {
+  return launch_finalize_pairs(pairs, meta, 0, grid, max_pairs, stream);
 }
 
 }  // namespace ssim2

@@ -39,6 +39,8 @@ constexpr int kTileCfgH[kNumTileCfgs] = {16, 16, 32, 16};
 enum InputKind : int {
   kInputSrgbU8 = 0,     // packed RGB u8, pitched
   kInputLinearF32 = 1,  // packed RGB f32 linear, pitched
+  kInputYuvP10 = 2,     // planar 4:2:0, u16 10-bit LSB-aligned: Y, Cb (c0), Cr (c1)
+  kInputYuvP016 = 3,    // biplanar 4:2:0, u16 MSB-aligned: Y, interleaved CbCr (c0)
 };
 
 // Which sides a launch processes.
@@ -71,6 +73,11 @@ struct PairDesc {
   float* lin_ref_out;           // optional packed f32 linear copy of the scale-0 input
   float* lin_dis_out;
   long long lin_ref_pitch, lin_dis_pitch;
+  const unsigned char* in_ref_c0;  // YUV chroma planes (see InputKind)
+  const unsigned char* in_ref_c1;
+  const unsigned char* in_dis_c0;
+  const unsigned char* in_dis_c1;
+  long long in_ref_cpitch, in_dis_cpitch;
   int in_kind;
   int nscales;
   int w, h;
@@ -90,8 +97,22 @@ struct InputArgs {
   float* lin_ref_out;
   float* lin_dis_out;
   long long lin_ref_pitch, lin_dis_pitch;
+  const unsigned char* ref_c0;
+  const unsigned char* ref_c1;
+  const unsigned char* dis_c0;
+  const unsigned char* dis_c1;
+  long long ref_cpitch, dis_cpitch;
   int kind;
   int use;  // 0: take inputs from the PairDesc
+};
+
+// Linear RGB of input pixel (gx, gy) of one image, for every InputKind.
+struct InputView {
+  const unsigned char* p;   // packed RGB, or the Y plane
+  const unsigned char* c0;  // Cb plane, or interleaved CbCr
+  const unsigned char* c1;  // Cr plane
+  long long pitch, cpitch;
+  int kind;
 };
 
 // One work item per block: {pair, a, b, c}; the meaning depends on the kernel.
@@ -117,7 +138,15 @@ hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* pr
                                int grid, hipStream_t stream);
 hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, const int* prefix,
                             const int* meta, int grid, hipStream_t stream);
-hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, hipStream_t stream);
+hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, int max_pairs,
+                                hipStream_t stream);
+// Shared finalisation (ssim2_finalize.hip): per-(pair, scale, channel)
+// reduction then the score; `meta` (device, may be null) holds the pair count,
+// else `npairs_static` is used.
+hipError_t launch_finalize_pairs(const PairDesc* pairs, const int* meta, int npairs_static, int grid_units,
+                                 int max_pairs, hipStream_t stream);
+hipError_t launch_yuv_to_linear(const InputView& in, float* out, long long out_pitch, int w, int h,
+                                hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream);
 hipError_t launch_srgb_to_linear(const unsigned char* in, long long in_pitch, float* out,
                                  long long out_pitch, int w, int h, hipStream_t stream);

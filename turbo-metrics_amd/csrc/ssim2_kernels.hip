@@ -20,6 +20,7 @@
 // Arithmetic follows crates/ssimulacra2-cuda/examples/cpu.rs exactly (see
 // ssim2_math.h), so every f32 plane equals the CPU's bit for bit and the score
 // differs only by the order of the f64 sums.
+#include "ssim2_input.h"
 #include "ssim2_internal.h"
 #include "ssim2_math.h"
 #include "ssim2_tables.h"
@@ -125,6 +126,12 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
     in.lin_dis_out = P.lin_dis_out;
     in.lin_ref_pitch = P.lin_ref_pitch;
     in.lin_dis_pitch = P.lin_dis_pitch;
+    in.ref_c0 = P.in_ref_c0;
+    in.ref_c1 = P.in_ref_c1;
+    in.dis_c0 = P.in_dis_c0;
+    in.dis_c1 = P.in_dis_c1;
+    in.ref_cpitch = P.in_ref_cpitch;
+    in.dis_cpitch = P.in_dis_cpitch;
     in.kind = P.in_kind;
   }
 
@@ -135,14 +142,9 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
     if (gx >= w || gy >= h) continue;
     for (int im = 0; im < NI; ++im) {
       const int img = SIDES == kSidesDis ? 1 : im;
-      gcuchar* base = gbl(img == 0 ? in.ref : in.dis);
-      const long long pitch = img == 0 ? in.ref_pitch : in.dis_pitch;
       float rgb[3];
-      if (in.kind == kInputSrgbU8) {
-        gcuchar* px = base + gy * pitch + gx * 3;
-        rgb[0] = c_lut[px[0]];
-        rgb[1] = c_lut[px[1]];
-        rgb[2] = c_lut[px[2]];
+      read_linear(input_of(in, img), gx, gy, c_lut, rgb);
+      if (in.kind != kInputLinearF32) {
         float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
         if (lo) {
           const long long lp = img == 0 ? in.lin_ref_pitch : in.lin_dis_pitch;
@@ -151,11 +153,6 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
           q[1] = rgb[1];
           q[2] = rgb[2];
         }
-      } else {
-        gcfloat* px = (gcfloat*)(base + gy * pitch) + gx * 3;
-        rgb[0] = px[0];
-        rgb[1] = px[1];
-        rgb[2] = px[2];
       }
       float X, Y, B;
       linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
@@ -266,21 +263,24 @@ __device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __
     load(0, a, b);
     quant(a, b, 0, R);  // columns 0..3 -> slots 0..3
   }
-  float4 qa[4], qb[4];
+  // Prefetch queue: slot j holds the raw inputs of columns n0+8..n0+11 of the
+  // group that consumes it, loaded kDepth groups earlier.
+  constexpr int kDepth = 8;
+  float4 qa[kDepth], qb[kDepth];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) load(4 + 4 * j, qa[j], qb[j]);
+  for (int j = 0; j < kDepth; ++j) load(4 + 4 * j, qa[j], qb[j]);
   float prev = 0.0f, prev2 = 0.0f;
   // Groups of four outputs cover n = -4 .. w-1; the loop runs whole sets of
-  // four groups so the body is straight-line (groups past the end are masked).
+  // kDepth groups so the body is straight-line (groups past the end are masked).
   // The three-term sum of group g-1 is read back from LDS while group g's
   // recurrence runs (software pipelining): a wave's LDS operations execute in
   // issue order, so only compiler reordering has to be prevented.
-  const int ngroups = (w + kRadius - 1 + 3) / 4;
-  const int nloop = (ngroups + 3) / 4 + 1;  // one extra set drains the last sum
+  const int ngroups = (w + kRadius - 1 + 3) / 4 + 1;  // one extra group drains the last sum
+  const int nloop = (ngroups + kDepth - 1) / kDepth;
   const int l1 = (lane + 1) & 63, l2 = (lane + 2) & 63;
-  for (int g0 = 0; g0 < 4 * nloop; g0 += 4) {
+  for (int g0 = 0; g0 < kDepth * nloop; g0 += kDepth) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < kDepth; ++j) {
       const int n0 = -kRadius + 1 + 4 * (g0 + j);
       const int base = (12 + 4 * j) & 15;  // n0 mod 16
       // 1. issue the reads of the previous group's terms
@@ -319,7 +319,7 @@ __device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __
       // hoisted the load above it, the slot would need two registers and the
       // loop back-edge copy would wait for the load (vmcnt(0)).
       __builtin_amdgcn_sched_barrier(0);
-      load(n0 + 24, qa[j], qb[j]);
+      load(n0 + 8 + 4 * kDepth, qa[j], qb[j]);
     }
   }
 }
@@ -493,59 +493,6 @@ __global__ __launch_bounds__(kThreads) void k_cols(const PairDesc* __restrict__ 
   }
 }
 
-// ---------------------------------------------------------------- finalize
-
-// One block of 1024 threads per pair. Wave w reduces the partial records of
-// (scale, channel) groups w and w+16 in a fixed order (strided lane sums, then
-// a shuffle tree), so the result does not depend on timing. Thread 0 then
-// forms the 108 features and the score (cpu.rs:633-634, 676-679, 840-868).
-__global__ __launch_bounds__(1024) void k_finalize(const PairDesc* __restrict__ pairs) {
-  const PairDesc& P = pairs[blockIdx.x];
-  __shared__ double sums[kSumsPerPair];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int g = wv; g < kMaxScales * 3; g += 16) {
-    const int s = g / 3, c = g % 3;
-    double v[kNumSums] = {0, 0, 0, 0, 0, 0};
-    if (s < P.nscales) {
-      const ScaleDesc& S = P.sc[s];
-      const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums;
-      for (int b = lane; b < S.colblocks; b += 64)
-#pragma unroll
-        for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
-    }
-#pragma unroll
-    for (int q = 0; q < kNumSums; ++q) {
-      const double r = wave_sum(v[q]);
-      if (lane == 0) {
-        sums[g * kNumSums + q] = r;
-        P.result[1 + g * kNumSums + q] = r;
-      }
-    }
-  }
-  if (t < 2 * kMaxScales) {
-    const int s = t / 2;
-    P.result[1 + kSumsPerPair + t] = s < P.nscales ? (double)(t & 1 ? P.sc[s].h : P.sc[s].w) : 0.0;
-  }
-  lds_barrier();
-  if (t != 0) return;
-  double ssim = 0.0;
-  int i = 0;
-  for (int c = 0; c < 3; ++c)
-    for (int s = 0; s < P.nscales; ++s) {
-      const double opp = 1.0 / (double)((long long)P.sc[s].w * P.sc[s].h);
-      const double* q = sums + s * 18 + c * kNumSums;
-      const double l1[3] = {opp * q[0], opp * q[2], opp * q[4]};
-      const double l4[3] = {sqrt(sqrt(opp * q[1])), sqrt(sqrt(opp * q[3])), sqrt(sqrt(opp * q[5]))};
-      for (int m = 0; m < 3; ++m) ssim = fma(c_weight[i++], fabs(l1[m]), ssim);
-      for (int m = 0; m < 3; ++m) ssim = fma(c_weight[i++], fabs(l4[m]), ssim);
-    }
-  ssim *= 0.9562382616834844;
-  ssim = fma(6.248496625763138e-5 * ssim * ssim, ssim,
-             fma(2.326765642916932, ssim, -0.020884521182843837 * ssim * ssim));
-  ssim = ssim > 0.0 ? fma(pow(ssim, 0.6276336467831387), -10.0, 100.0) : 100.0;
-  P.result[0] = ssim;
-}
-
 // ---------------------------------------------------------------- utilities
 
 // crates/ssimulacra2-cuda-kernel/src/srgb.rs:58-70 (Kernel::srgb_to_linear):
@@ -566,6 +513,21 @@ __global__ __launch_bounds__(kThreads) void k_copy(const uint4* __restrict__ src
                                                    long long n16) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+// crates/cuda-colorspace-kernel/src/biplanar.rs:8-70 (biplanaryuv420_to_linearrgb):
+// YUV 4:2:0 -> packed f32 linear RGB, one thread per pixel.
+__global__ __launch_bounds__(kThreads) void k_yuv_to_linear(InputView in, float* __restrict__ out,
+                                                            long long out_pitch, int w, int h) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  float rgb[3];
+  read_linear(in, x, y, c_lut, rgb);
+  gfloat* o = gbl(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(out) + y * out_pitch)) + x * 3;
+  o[0] = rgb[0];
+  o[1] = rgb[1];
+  o[2] = rgb[2];
 }
 
 // ---------------------------------------------------------------- launchers
@@ -603,8 +565,7 @@ hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems,
 
 hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream) {
   if (npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_finalize, dim3(npairs), dim3(1024), 0, stream, pairs);
-  return hipGetLastError();
+  return launch_finalize_pairs(pairs, nullptr, npairs, npairs * kMaxScales * 3, npairs, stream);
 }
 
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream) {
@@ -613,6 +574,14 @@ hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t 
   const long long blocks = std::min<long long>((n16 + kThreads - 1) / kThreads, 8192);
   hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(kThreads), 0, stream,
                      static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_yuv_to_linear(const InputView& in, float* out, long long out_pitch, int w, int h,
+                                hipStream_t stream) {
+  if (w <= 0 || h <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_yuv_to_linear, dim3((w + kThreads - 1) / kThreads, h), dim3(kThreads), 0, stream, in, out,
+                     out_pitch, w, h);
   return hipGetLastError();
 }
 

@@ -94,4 +94,32 @@ SSIM2_HD double edge_ratio(float src, float mu1, float dis, float mu2) {
   return (1.0 + (double)fabsf(dis - mu2)) / (1.0 + (double)fabsf(src - mu1)) - 1.0;
 }
 
+// ---- YUV 4:2:0 BT.709 limited range -> linear RGB ------------------------
+// crates/cuda-colorspace-kernel/src/biplanar.rs:8-70 with lib.rs:205-246:
+// 16-bit-range samples (10-bit values shifted left by 6), luma floor 16<<8,
+// chroma neutral 1<<15, Kr/Kb derived from the BT.709 primaries and D65, one
+// chroma sample per 2x2 quad, BT.709 inverse OETF, clamped to [0, 1]. The
+// coefficients are the f32 values of that derivation
+// (y = 1/56064, r = 2(1-Kr)/57344, b = 2(1-Kb)/57344,
+//  g1 = -2(1-Kb)Kb/Kg/57344, g2 = -2(1-Kr)Kr/Kg/57344).
+constexpr float kYuvY = 0x1.2b404ap-16f, kYuvR = 0x1.ccb7fap-16f, kYuvB = 0x1.0f7334p-15f;
+constexpr float kYuvG1 = -0x1.b66c0ap-19f, kYuvG2 = -0x1.11f7e2p-17f;
+constexpr float kBt709AlphaM1 = 0x1.96b840p-4f, kBt709Alpha = 0x1.196b84p+0f;
+constexpr float kBt709Threshold = 0x1.4cc550p-4f, kBt709InvGamma = 0x1.1c71c8p+1f;
+
+SSIM2_HD float bt709_eotf(float v) {
+  return v >= kBt709Threshold ? powf((v + kBt709AlphaM1) / kBt709Alpha, kBt709InvGamma) : v / 4.5f;
+}
+
+SSIM2_HD void yuv16_to_linear(unsigned y, unsigned u, unsigned v, float* rgb) {
+  const float cb = (float)((int)u - 32768), cr = (float)((int)v - 32768);
+  const float r_ = kYuvR * cr;
+  const float g_ = fmaf(kYuvG1, cb, kYuvG2 * cr);
+  const float b_ = kYuvB * cb;
+  const float luma = (float)((y > 4096u ? y : 4096u) - 4096u) * kYuvY;
+  rgb[0] = fminf(fmaxf(bt709_eotf(luma + r_), 0.0f), 1.0f);
+  rgb[1] = fminf(fmaxf(bt709_eotf(luma + g_), 0.0f), 1.0f);
+  rgb[2] = fminf(fmaxf(bt709_eotf(luma + b_), 0.0f), 1.0f);
+}
+
 }  // namespace ssim2

@@ -73,6 +73,27 @@ class _Pair(ctypes.Structure):
     _fields_ = [("ref", _Image), ("dis", _Image)]
 
 
+class _Yuv(ctypes.Structure):
+    _fields_ = [
+        ("y", ctypes.c_void_p),
+        ("c0", ctypes.c_void_p),
+        ("c1", ctypes.c_void_p),
+        ("y_pitch_bytes", ctypes.c_size_t),
+        ("c_pitch_bytes", ctypes.c_size_t),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("format", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+    ]
+
+
+class _YuvPair(ctypes.Structure):
+    _fields_ = [("ref", _Yuv), ("dis", _Yuv)]
+
+
+YUV_FORMATS = {"yuv420p10": 0, "p016": 1}
+
+
 class _BatchOptions(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32),
@@ -160,6 +181,13 @@ def _load():
         "ssim2_has_reference": (vp, P(ctypes.c_int)),
         "ssim2_get_sums": (vp, P(dbl), P(ctypes.c_int), P(ctypes.c_int)),
         "ssim2_srgb_to_linear": (img, img, vp),
+        "ssim2_yuv420_to_linear": (P(_Yuv), img, vp),
+        "ssim2_compute_yuv420_async": (vp, P(_Yuv), P(_Yuv), vp),
+        "ssim2_compute_yuv420_sync": (vp, P(_Yuv), P(_Yuv), vp, P(dbl)),
+        "ssim2_yuv_alloc": (u32, u32, u32, P(_Yuv)),
+        "ssim2_yuv_free": (P(_Yuv),),
+        "ssim2_yuv_upload": (P(_Yuv), vp, vp, vp),
+        "ssim2_batch_compute_yuv_async": (vp, P(_YuvPair), u32, vp),
         "ssim2_image_alloc": (u32, u32, u32, img),
         "ssim2_image_free": (img,),
         "ssim2_image_upload": (img, vp, sz),
@@ -301,6 +329,64 @@ class DeviceImage:
             pass
 
 
+class DeviceYuv:
+    """YUV 4:2:0 frame in device memory, BT.709 limited range, u16 samples
+    (the decoded surfaces of crates/cudarse/cudarse-video/src/dec.rs).
+    format "yuv420p10": planar Y, Cb, Cr with 10-bit LSB-aligned samples;
+    "p016": Y plus interleaved CbCr with samples in the high bits."""
+
+    def __init__(self, width: int, height: int, fmt: str = "yuv420p10"):
+        self.fmt = fmt
+        self._img = _Yuv()
+        _check(lib.ssim2_yuv_alloc(width, height, YUV_FORMATS[fmt], ctypes.byref(self._img)))
+
+    @property
+    def width(self):
+        return self._img.width
+
+    @property
+    def height(self):
+        return self._img.height
+
+    def as_struct(self):
+        return ctypes.byref(self._img)
+
+    @classmethod
+    def from_planes(cls, y: np.ndarray, cb: np.ndarray, cr: np.ndarray, fmt: str = "yuv420p10") -> "DeviceYuv":
+        """Upload 10-bit planes (u16, LSB-aligned). For "p016" the samples are
+        shifted to the high bits and Cb/Cr interleaved on the host."""
+        y = np.ascontiguousarray(y, dtype=np.uint16)
+        h, w = y.shape
+        cw, ch = (w + 1) // 2, (h + 1) // 2
+        cb = np.ascontiguousarray(cb, dtype=np.uint16)
+        cr = np.ascontiguousarray(cr, dtype=np.uint16)
+        if cb.shape != (ch, cw) or cr.shape != (ch, cw):
+            raise ValueError(f"chroma planes must be {(ch, cw)}")
+        im = cls(w, h, fmt)
+        if fmt == "p016":
+            yy = np.ascontiguousarray(y << 6)
+            uv = np.ascontiguousarray(np.stack([cb << 6, cr << 6], axis=-1))
+            _check(lib.ssim2_yuv_upload(ctypes.byref(im._img), yy.ctypes.data, uv.ctypes.data, None))
+        else:
+            _check(lib.ssim2_yuv_upload(ctypes.byref(im._img), y.ctypes.data, cb.ctypes.data, cr.ctypes.data))
+        return im
+
+    def free(self):
+        if self._img.y:
+            lib.ssim2_yuv_free(ctypes.byref(self._img))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def yuv420_to_linear(src: DeviceYuv, dst: DeviceImage, stream: Stream | None = None):
+    """CudaColorspace::biplanaryuv420_to_linearrgb_16 (crates/cuda-colorspace/src/lib.rs:79-123)."""
+    _check(lib.ssim2_yuv420_to_linear(src.as_struct(), dst.as_struct(), _stream_ptr(stream)))
+
+
 def device_synchronize():
     """hipDeviceSynchronize."""
     _check(lib.ssim2_device_synchronize())
@@ -415,6 +501,16 @@ class Ssimulacra2:
         _check(lib.ssim2_compute_srgb_async(self._ctx, src_ref.as_struct(), src_dis.as_struct(),
                                             _stream_ptr(stream)))
 
+    def compute_yuv420(self, ref: DeviceYuv, dis: DeviceYuv, stream=None):
+        """Async score of a YUV 4:2:0 frame pair (conversion fused into the first kernel)."""
+        _check(lib.ssim2_compute_yuv420_async(self._ctx, ref.as_struct(), dis.as_struct(), _stream_ptr(stream)))
+
+    def compute_yuv420_sync(self, ref: DeviceYuv, dis: DeviceYuv, stream=None) -> float:
+        out = ctypes.c_double()
+        _check(lib.ssim2_compute_yuv420_sync(self._ctx, ref.as_struct(), dis.as_struct(), _stream_ptr(stream),
+                                             ctypes.byref(out)))
+        return out.value
+
     def compute_from_cpu_srgb_sync(self, ref_bytes: np.ndarray, dis_bytes: np.ndarray, stream=None) -> float:
         r = np.ascontiguousarray(ref_bytes, dtype=np.uint8)
         d = np.ascontiguousarray(dis_bytes, dtype=np.uint8)
@@ -525,6 +621,19 @@ class Ssim2Batch:
         arr = self._pairs(pairs)
         self._last = (arr, len(pairs))
         _check(lib.ssim2_batch_compute_async(self._b, arr, len(pairs), _stream_ptr(stream)))
+
+    def compute_yuv_async(self, pairs, stream=None):
+        """pairs: sequence of (ref DeviceYuv, dis DeviceYuv)."""
+        arr = (_YuvPair * len(pairs))()
+        for i, (r, d) in enumerate(pairs):
+            arr[i].ref = r._img
+            arr[i].dis = d._img
+        self._last = (arr, len(pairs))
+        _check(lib.ssim2_batch_compute_yuv_async(self._b, arr, len(pairs), _stream_ptr(stream)))
+
+    def compute_yuv(self, pairs, stream=None):
+        self.compute_yuv_async(pairs, stream)
+        return self.get_scores()
 
     def get_scores(self):
         n = self._last[1]
