@@ -18,6 +18,13 @@ The same JSON line also carries
                 a bounded sample; the same leg checks the fast-form scores
                 against the FP64 oracle (accuracy columns).
 
+  video         3840x2160 10-bit BT.709 4:2:0 frame pairs (value noise drifting
+                2 px/frame vs 6-bit luma banding + 3x3 chroma blur), converted
+                on the GPU inside the first kernel; exact f32 path per frame
+                and the batched fast form with f32/f16/bf16 storage, |dscore|
+                against the FP64 oracle per frame. By default 24 of the 240
+                frames (every 10th); `--video-frames 240` runs all of them.
+
 `--workload serving --serving-seconds 60` runs the full serving workload alone.
 """
 import argparse
@@ -104,8 +111,15 @@ def run_plain(s2, synth, steps, warmup):
         lat.append((time.perf_counter() - t) * 1e3)
     lat.sort()
     kms = m.time_kernels(dr, dd, reps=max(5, min(steps, 50)), stream=stream)
+    # host bytes in (compute_from_cpu_srgb_sync): H2D of both images + compute + sync
+    n_host = max(5, min(steps, 50))
+    assert m.compute_from_cpu_srgb_sync(ref, dis, stream) == score
+    t0 = time.perf_counter()
+    for _ in range(n_host):
+        m.compute_from_cpu_srgb_sync(ref, dis, stream)
+    host_dt = (time.perf_counter() - t0) / n_host
     m.close()
-    return {"ref": ref, "dis": dis, "score": score, "dt": dt, "lat": lat, "kms": kms}
+    return {"ref": ref, "dis": dis, "score": score, "dt": dt, "lat": lat, "kms": kms, "host_dt": host_dt}
 
 
 def run_batch(s2, workloads, storage, host_pairs, spec, reps):
@@ -198,6 +212,131 @@ def run_serving(s2, workloads, seconds, rate, storage="f16", slots=3):
     }
 
 
+def run_mixed(s2, workloads, pool_dev, spec, cls, seconds=1.0, rate=2000.0, reps=3, storage="f16"):
+    """Throughput on the mixed-size request stream (the first `seconds` of the
+    Poisson schedule, all requests present at once): the plain exact f32 path
+    pair by pair (one context per size, async on one stream) against the
+    batched fast form with up to 512 pairs / 64 Mpx per launch."""
+    per_class = [cls.count(c) for c in range(4)]
+    _, pick = workloads.serving_schedule(rate, seconds, pool_per_class=per_class)
+    reqs = [int(k) for k in pick]
+    px = sum(spec[k][2] * spec[k][3] for k in reqs)
+    stream = s2.Stream()
+    ctxs = {}
+    for k in set(reqs):
+        wh = (spec[k][2], spec[k][3])
+        if wh not in ctxs:
+            ctxs[wh] = s2.Ssimulacra2(*wh)
+    plain_scores = {}
+    for k in set(reqs):  # warm-up, one score per pool pair
+        c = ctxs[(spec[k][2], spec[k][3])]
+        plain_scores[k] = c.compute_srgb_sync(*pool_dev[k], stream=stream)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for k in reqs:
+            ctxs[(spec[k][2], spec[k][3])].compute_srgb(*pool_dev[k], stream)
+    stream.sync()
+    dt_plain = (time.perf_counter() - t0) / reps
+    for c in ctxs.values():
+        c.close()
+    # batches as the server forms them: FIFO, at most 512 pairs and 64 Mpx
+    batches, cur, cur_px = [], [], 0
+    for k in reqs:
+        p = spec[k][2] * spec[k][3]
+        if cur and (len(cur) == 512 or cur_px + p > (64 << 20)):
+            batches.append(cur)
+            cur, cur_px = [], 0
+        cur.append(k)
+        cur_px += p
+    if cur:
+        batches.append(cur)
+    b = s2.Ssim2Batch(max_pairs=512, max_pixels=64 << 20, storage=storage)
+    b.autotune([pool_dev[k] for k in batches[0]], reps=2, stream=stream)
+    for bt in batches:
+        b.compute([pool_dev[k] for k in bt], stream)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for bt in batches:
+            b.compute_async([pool_dev[k] for k in bt], stream)
+            b.get_scores()
+    dt_batch = (time.perf_counter() - t0) / reps
+    b.close()
+    plain = px / 1e6 / dt_plain
+    batched = px / 1e6 / dt_batch
+    return {"requests": len(reqs), "mpx": round(px / 1e6, 1), "batches": len(batches),
+            "plain_exact_f32": round(plain, 1), f"batched_{storage}": round(batched, 1),
+            "unit": "Mpx/s", "ratio": round(batched / plain, 2)}
+
+
+VIDEO_W, VIDEO_H, VIDEO_SEED, VIDEO_LEN = 3840, 2160, 3, 240
+
+
+def video_frames(n):
+    """Frame indices of an n-frame sample spread over the 240-frame sequence."""
+    n = max(1, min(VIDEO_LEN, n))
+    return [int(t) for t in np.linspace(0, VIDEO_LEN, n, endpoint=False).round()]
+
+
+def run_video(s2, synth, nframes, reps=3):
+    ts = video_frames(nframes)
+    with ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
+        host = list(ex.map(lambda t: synth.yuv420_pair(VIDEO_SEED, VIDEO_W, VIDEO_H, t), ts))
+    dev = [(s2.DeviceYuv.from_planes(*r), s2.DeviceYuv.from_planes(*d)) for r, d in host]
+    px = VIDEO_W * VIDEO_H * len(dev)
+    stream = s2.Stream()
+    m = s2.Ssimulacra2(VIDEO_W, VIDEO_H)
+    exact = []
+    for r, d in dev:  # warm-up and scores
+        exact.append(m.compute_yuv420_sync(r, d, stream))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for r, d in dev:
+            m.compute_yuv420(r, d, stream)
+    stream.sync()
+    dt_exact = (time.perf_counter() - t0) / reps
+    m.close()
+    legs = {"exact_f32": {"value": round(px / 1e6 / dt_exact, 1), "unit": "Mpx/s",
+                          "ms_per_frame": round(dt_exact / len(dev) * 1e3, 4), "scores": exact}}
+    for storage in ("f32", "f16", "bf16"):
+        b = s2.Ssim2Batch(max_pairs=len(dev), max_pixels=px + 1024, storage=storage)
+        scores = b.compute_yuv(dev, stream)
+        stream.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.compute_yuv_async(dev, stream)
+        stream.sync()
+        dt = (time.perf_counter() - t0) / reps
+        assert np.array_equal(b.get_scores(), scores)
+        b.close()
+        legs[f"batched_{storage}"] = {"value": round(px / 1e6 / dt, 1), "unit": "Mpx/s",
+                                      "ms_per_frame": round(dt / len(dev) * 1e3, 4), "scores": list(scores)}
+    del dev
+    return {"frames": ts, "width": VIDEO_W, "height": VIDEO_H, "format": "yuv420p10", "legs": legs, "host": host}
+
+
+def video_accuracy(video):
+    """|dscore| of every video leg against the FP64 oracle, per frame (checker only)."""
+    oracle = ctypes.CDLL(str(ROOT / "oracle" / "libssim2_oracle.so"))
+    vp, i = ctypes.c_void_p, ctypes.c_int
+    oracle.oracle_score_yuv420p10_f64.argtypes = [vp] * 6 + [i, i, ctypes.POINTER(ctypes.c_double)]
+
+    def f64(pair):
+        planes = [np.ascontiguousarray(p) for p in (*pair[0], *pair[1])]
+        s = ctypes.c_double()
+        oracle.oracle_score_yuv420p10_f64(*[p.ctypes.data for p in planes], VIDEO_W, VIDEO_H, ctypes.byref(s))
+        return s.value
+
+    # about 2 GB of host memory per 4K FP64 evaluation
+    with ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
+        ref = list(ex.map(f64, video["host"]))
+    video["f64_scores"] = [round(v, 6) for v in ref]
+    for leg in video["legs"].values():
+        d = [abs(a - b) for a, b in zip(leg["scores"], ref)]
+        leg["abs_delta_vs_f64"] = [round(x, 5) for x in d]
+        leg["max_abs_delta_vs_f64"] = round(max(d), 5)
+        leg["scores"] = [round(x, 6) for x in leg["scores"]]
+
+
 # ---------------------------------------------------------------- CPU leg (oracle)
 
 def cpu_leg(plain, batch_legs, serving, batch_host, batch_spec):
@@ -279,10 +418,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["all", "plain", "serving"], default="all")
+    ap.add_argument("--workload", choices=["all", "plain", "batch", "serving", "video"], default="all")
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--serving-rate", type=float, default=2000.0)
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--video-frames", type=int, default=24,
+                    help="frames of the 240-frame 4K YUV sequence to run (0 skips the video leg)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run the plain copy kernel once over this many MiB (counter calibration)")
@@ -307,6 +448,26 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
+    if args.workload == "batch":
+        bspec = workloads.uneven_batch_spec()
+        bhost = workloads.generate(bspec)
+        legs = [run_batch(s2, workloads, st, bhost, bspec, args.batch_reps) for st in ("f16", "f32")]
+        print(json.dumps({"metric": "ssimulacra2 Mpx/s, 256 uneven pairs per launch", "n_gpus": 1,
+                          "legs": [{k: v for k, v in leg.items() if k != "scores"} for leg in legs]}), flush=True)
+        return
+
+    if args.workload == "video":
+        log(f"video: {args.video_frames} frames of 3840x2160 yuv420p10")
+        video = run_video(s2, synth, args.video_frames)
+        if not args.no_cpu_baseline:
+            log("video accuracy against the FP64 oracle")
+            video_accuracy(video)
+        out = {k: v for k, v in video.items() if k != "host"}
+        out.update({"metric": "ssimulacra2 Mpx/s on 4K 10-bit YUV 4:2:0 frames", "n_gpus": 1,
+                    "higher_is_better": True, "data": "synthetic"})
+        print(json.dumps(out), flush=True)
+        return
+
     log("plain 1920x1080 exact path")
     plain = run_plain(s2, synth, args.steps, args.warmup)
     alg = algorithmic_bytes(W, H)
@@ -314,7 +475,7 @@ def main():
     dom = max(("pyramid", "rows", "cols"), key=lambda k: kms[k])
     achieved = alg[dom] / (kms[dom] * 1e-3) / 1e9
 
-    batch_legs, serving, bhost, bspec = [], None, None, None
+    batch_legs, serving, bhost, bspec, mixed = [], None, None, None, None
     if args.workload == "all":
         bspec = workloads.uneven_batch_spec()
         log(f"generating the uneven batch ({workloads.total_pixels(bspec) / 1e6:.1f} Mpx)")
@@ -324,6 +485,15 @@ def main():
             batch_legs.append(run_batch(s2, workloads, storage, bhost, bspec, args.batch_reps))
         log(f"serving {args.serving_seconds} s at {args.serving_rate} req/s")
         serving = run_serving(s2, workloads, args.serving_seconds, args.serving_rate)
+        log("mixed-size stream: plain exact path vs batched fast form")
+        pool_dev = [(s2.DeviceImage.from_host(r), s2.DeviceImage.from_host(d)) for r, d in serving["pool_host"]]
+        spec, cls = workloads.serving_pool_spec()
+        mixed = run_mixed(s2, workloads, pool_dev, spec, cls)
+        del pool_dev
+    video = None
+    if args.workload == "all" and args.video_frames > 0:
+        log(f"video: {args.video_frames} frames of 3840x2160 yuv420p10")
+        video = run_video(s2, synth, args.video_frames)
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy * (1 << 20)
@@ -358,6 +528,9 @@ def main():
         "latency_ms": {"p50": round(statistics.median(plain["lat"]), 4),
                        "p99": round(plain["lat"][int(0.99 * (len(plain["lat"]) - 1))], 4)},
         "kernels_ms": {k: round(v, 5) for k, v in kms.items()},
+        "pcie_inclusive": {"value": round(mpx / plain["host_dt"], 1), "unit": "Mpx/s",
+                           "ms_per_pair": round(plain["host_dt"] * 1e3, 4),
+                           "note": "host u8 bytes in, H2D + compute + sync per pair (not the headline)"},
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -375,10 +548,16 @@ def main():
         cb, acc = cpu_leg(plain, batch_legs, serving, bhost, bspec)
         out["cpu_baseline"] = cb
         out["accuracy"] = acc
+        if video:
+            video_accuracy(video)
     if batch_legs:
         out["batched_fast"] = [{k: v for k, v in leg.items() if k != "scores"} for leg in batch_legs]
     if serving:
         out["serving"] = {k: v for k, v in serving.items() if not k.startswith("pool_")}
+    if mixed:
+        out["mixed_stream"] = mixed
+    if video:
+        out["video"] = {k: v for k, v in video.items() if k != "host"}
     print(json.dumps(out), flush=True)
 
 

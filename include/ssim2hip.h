@@ -168,6 +168,17 @@ ssim2_status ssim2_compute_with_reference_linear(ssim2_ctx* ctx, const ssim2_ima
 ssim2_status ssim2_compute_with_reference_srgb(ssim2_ctx* ctx, const ssim2_image* dis_srgb,
                                                ssim2_stream stream, double* score);
 
+/* Host-byte forms of the cached-reference flow:
+ * Ssimulacra2::prepare_reference_from_cpu_srgb (lib.rs:373-388) uploads the
+ * reference bytes into the context's staging image and caches its state;
+ * Ssimulacra2::compute_from_cpu_dis_only_srgb_sync (lib.rs:347-371) uploads
+ * only the distorted bytes and scores them against the cached reference
+ * (SSIM2_NO_REFERENCE without one). */
+ssim2_status ssim2_prepare_reference_from_host_srgb(ssim2_ctx* ctx, const uint8_t* ref, size_t row_bytes,
+                                                    ssim2_stream stream);
+ssim2_status ssim2_compute_from_host_dis_srgb(ssim2_ctx* ctx, const uint8_t* dis, size_t row_bytes,
+                                              ssim2_stream stream, double* score);
+
 /* Ssimulacra2::clear_reference / has_reference (lib.rs:592-602). */
 ssim2_status ssim2_clear_reference(ssim2_ctx* ctx);
 ssim2_status ssim2_has_reference(const ssim2_ctx* ctx, int* has_reference);

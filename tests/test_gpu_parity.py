@@ -126,6 +126,23 @@ def test_cached_reference_equals_direct(s2, synth):
     assert again == direct[1] and cached2 == direct[0]
 
 
+def test_host_bytes_cached_reference(s2, synth):
+    """prepare_reference_from_cpu_srgb + compute_from_cpu_dis_only_srgb_sync
+    (lib.rs:347-388) give the direct scores."""
+    w, h = 200, 150
+    ref, d1 = synth.pair(2, 43, w, h, 12)
+    _, d2 = synth.pair(2, 44, w, h, 30)
+    _say("[gpu] host-bytes cached reference 200x150")
+    with s2.Ssimulacra2(w, h) as m:
+        direct = [m.compute_from_cpu_srgb_sync(ref, d) for d in (d1, d2)]
+        with pytest.raises(s2.Ssim2Error) as ei:
+            m.compute_from_cpu_dis_only_srgb_sync(d1)
+        assert ei.value.status == "SSIM2_NO_REFERENCE"
+        m.prepare_reference_from_cpu_srgb(ref)
+        cached = [m.compute_from_cpu_dis_only_srgb_sync(d) for d in (d1, d2)]
+    assert cached == direct
+
+
 def test_repeated_and_interleaved_sizes(s2, oracle, synth):
     """Contexts of several sizes alive at once, reused (drop_stress.rs:31-128 intent)."""
     sizes = [(96, 64), (257, 129), (64, 200)]

@@ -6,6 +6,7 @@ transposed recurrence form, a dropped FIR tap, a wrong scale count, a
 mis-ordered weight. None re-types the oracle's formulas.
 """
 import json
+from pathlib import Path
 import math
 import struct
 
@@ -193,6 +194,23 @@ def test_cbrt_is_the_platform_cbrtf(oracle):
     k = np.float32(0.0037930734)
     approx = (np.cbrt((k + g).astype(np.float64)) - 0.1559542025327239) + 0.01
     assert np.abs(y - approx).max() < 3e-7
+
+
+def test_kernel_cbrt_equals_libm_on_every_float(tmp_path):
+    """cbrtf_ref (the GPU kernels' cube root, ssim2_math.h) restates glibc's
+    cbrtf, which cpu.rs reaches through f32::cbrt. Built here as host code and
+    compared with the host cbrtf on every float in [0, 2] (the XYB argument
+    range is [0.0038, 1.0038])."""
+    import os
+    import subprocess
+    exe = tmp_path / "cbrt_exhaustive"
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-pthread", "-D__HIP_PLATFORM_AMD__",
+                    "-I/opt/rocm/include", f"-I{ROOT / 'turbo-metrics_amd' / 'csrc'}",
+                    str(Path(__file__).parent / "cbrt_exhaustive.cpp"), "-o", str(exe), "-lm"], check=True)
+    threads = str(max(1, min(8, os.cpu_count() or 1)))
+    out = subprocess.run([str(exe), "0", "2", threads], check=True, capture_output=True, text=True).stdout
+    checked, differ = map(int, out.split())
+    assert checked == 0x40000000 + 1 and differ == 0
 
 
 def test_downscale_brute_force(oracle):

@@ -311,30 +311,63 @@ ssim2_status ssim2_compute_srgb_sync(ssim2_ctx* c, const ssim2_image* ref, const
   return ssim2_get_score(c, score);
 }
 
+ssim2_status ensure_stage(ssim2_ctx* c) {
+  if (c->stage) return SSIM2_OK;
+  c->stage_pitch = (size_t(c->w) * 3 + 255) / 256 * 256;
+  hipError_t e = hipMalloc(&c->stage, 2 * c->stage_pitch * size_t(c->h));
+  if (e != hipSuccess) {
+    c->stage = nullptr;
+    (void)hipGetLastError();
+    return fail(SSIM2_OOM, "staging allocation failed");
+  }
+  return SSIM2_OK;
+}
+
+// Uploads `host` into staging image `which` (0 ref, 1 dis) on `s`.
+ssim2_status upload_stage(ssim2_ctx* c, int which, const uint8_t* host, size_t row_bytes, hipStream_t s,
+                          ssim2_image* out) {
+  const size_t rb = size_t(c->w) * 3;
+  if (row_bytes < rb) return fail(SSIM2_INVALID_ARGUMENT, "row_bytes smaller than 3*width");
+  ssim2_status st = ensure_stage(c);
+  if (st != SSIM2_OK) return st;
+  unsigned char* p = c->stage + (which ? c->stage_pitch * size_t(c->h) : 0);
+  HIP_TRY(hipMemcpy2DAsync(p, c->stage_pitch, host, row_bytes, rb, c->h, hipMemcpyHostToDevice, s));
+  *out = ssim2_image{p, c->stage_pitch, (uint32_t)c->w, (uint32_t)c->h};
+  return SSIM2_OK;
+}
+
 ssim2_status ssim2_compute_from_host_srgb(ssim2_ctx* c, const uint8_t* ref, const uint8_t* dis,
                                           size_t row_bytes, ssim2_stream stream, double* score) {
   if (!c || !ref || !dis || !score) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
-  const size_t rb = size_t(c->w) * 3;
-  if (row_bytes < rb) return fail(SSIM2_INVALID_ARGUMENT, "row_bytes smaller than 3*width");
-  if (!c->stage) {
-    c->stage_pitch = (rb + 255) / 256 * 256;
-    hipError_t e = hipMalloc(&c->stage, 2 * c->stage_pitch * size_t(c->h));
-    if (e != hipSuccess) {
-      c->stage = nullptr;
-      (void)hipGetLastError();
-      return fail(SSIM2_OOM, "staging allocation failed");
-    }
-  }
   hipStream_t s = (hipStream_t)stream;
-  unsigned char* sr = c->stage;
-  unsigned char* sd = c->stage + c->stage_pitch * size_t(c->h);
-  HIP_TRY(hipMemcpy2DAsync(sr, c->stage_pitch, ref, row_bytes, rb, c->h, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpy2DAsync(sd, c->stage_pitch, dis, row_bytes, rb, c->h, hipMemcpyHostToDevice, s));
-  ssim2_image ri{sr, c->stage_pitch, (uint32_t)c->w, (uint32_t)c->h};
-  ssim2_image di{sd, c->stage_pitch, (uint32_t)c->w, (uint32_t)c->h};
-  ssim2_status st = enqueue(c, kSidesBoth, make_inputs(&ri, &di, kInputSrgbU8), s);
+  ssim2_image ri, di;
+  ssim2_status st = upload_stage(c, 0, ref, row_bytes, s, &ri);
+  if (st == SSIM2_OK) st = upload_stage(c, 1, dis, row_bytes, s, &di);
+  if (st != SSIM2_OK) return st;
+  st = enqueue(c, kSidesBoth, make_inputs(&ri, &di, kInputSrgbU8), s);
   if (st != SSIM2_OK) return st;
   return ssim2_get_score(c, score);
+}
+
+ssim2_status ssim2_prepare_reference_from_host_srgb(ssim2_ctx* c, const uint8_t* ref, size_t row_bytes,
+                                                    ssim2_stream stream) {
+  if (!c || !ref) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_image ri;
+  ssim2_status st = upload_stage(c, 0, ref, row_bytes, (hipStream_t)stream, &ri);
+  if (st != SSIM2_OK) return st;
+  return ssim2_set_reference_srgb(c, &ri, stream);
+}
+
+ssim2_status ssim2_compute_from_host_dis_srgb(ssim2_ctx* c, const uint8_t* dis, size_t row_bytes,
+                                              ssim2_stream stream, double* score) {
+  if (!c || !dis || !score) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  int has = 0;
+  ssim2_has_reference(c, &has);
+  if (!has) return fail(SSIM2_NO_REFERENCE, "no cached reference: call a set/prepare_reference entry first");
+  ssim2_image di;
+  ssim2_status st = upload_stage(c, 1, dis, row_bytes, (hipStream_t)stream, &di);
+  if (st != SSIM2_OK) return st;
+  return ssim2_compute_with_reference_srgb(c, &di, stream, score);
 }
 
 ssim2_status ssim2_compute_yuv420_async(ssim2_ctx* c, const ssim2_yuv_image* ref, const ssim2_yuv_image* dis,

@@ -7,14 +7,14 @@
 //              -> XYB planes of every scale. The linear pyramid never leaves
 //              LDS: a 32x32 tile at scale 0 owns exactly its 16x16 .. 1x1
 //              sub-tiles at scales 1..5, including the ceil(n/2) clamped edge.
-//   k_rows     one block per (scale, channel, 16-row band): the row-direction
-//              recursive Gaussian of cpu.rs (row form), 15 recurrences per row
-//              (5 quantities x 3 cosine terms), one lane per recurrence; inputs
-//              and products staged in LDS in 32-column chunks.
+//   k_rows     one wave per (scale, channel, quantity, band of 21 rows): the
+//              row-direction recursive Gaussian of cpu.rs (row form), one lane
+//              per (row, cosine term), inputs streamed from the XYB planes
+//              through a register prefetch queue.
 //   k_cols     one block per (scale, channel, 16-column strip): the column
 //              pass (column form), then SSIM / edge-difference error maps and
 //              f64 L1/L4 partial sums, all in the same block.
-//   k_finalize one block per pair: fixed-order reduction of the partial sums
+//   finalize   (ssim2_finalize.hip) fixed-order reduction of the partial sums
 //              and the 108-feature weighted score, in f64 on the device.
 //
 // Arithmetic follows crates/ssimulacra2-cuda/examples/cpu.rs exactly (see

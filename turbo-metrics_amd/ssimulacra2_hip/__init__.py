@@ -173,6 +173,8 @@ def _load():
         "ssim2_compute_srgb_sync": (vp, img, img, img, img, vp, P(dbl)),
         "ssim2_compute_srgb_async": (vp, img, img, vp),
         "ssim2_compute_from_host_srgb": (vp, vp, vp, sz, vp, P(dbl)),
+        "ssim2_prepare_reference_from_host_srgb": (vp, vp, sz, vp),
+        "ssim2_compute_from_host_dis_srgb": (vp, vp, sz, vp, P(dbl)),
         "ssim2_set_reference_linear": (vp, img, vp),
         "ssim2_set_reference_srgb": (vp, img, vp),
         "ssim2_compute_with_reference_linear": (vp, img, img, vp, P(dbl)),
@@ -523,6 +525,20 @@ class Ssimulacra2:
         return v.value
 
     # -- cached reference (lib.rs:440-602)
+    def prepare_reference_from_cpu_srgb(self, ref_bytes: np.ndarray, stream=None):
+        """Upload host sRGB bytes (h, w, 3) and cache them as the reference (lib.rs:373-388)."""
+        a = np.ascontiguousarray(ref_bytes, np.uint8)
+        _check(lib.ssim2_prepare_reference_from_host_srgb(self._ctx, a.ctypes.data, a.strides[0],
+                                                          _stream_ptr(stream)))
+
+    def compute_from_cpu_dis_only_srgb_sync(self, dis_bytes: np.ndarray, stream=None) -> float:
+        """Score host sRGB bytes against the cached reference (lib.rs:347-371)."""
+        a = np.ascontiguousarray(dis_bytes, np.uint8)
+        v = ctypes.c_double()
+        _check(lib.ssim2_compute_from_host_dis_srgb(self._ctx, a.ctypes.data, a.strides[0], _stream_ptr(stream),
+                                                    ctypes.byref(v)))
+        return v.value
+
     def set_reference_linear(self, ref_linear: DeviceImage, stream=None):
         _check(lib.ssim2_set_reference_linear(self._ctx, ref_linear.as_struct(), _stream_ptr(stream)))
 
