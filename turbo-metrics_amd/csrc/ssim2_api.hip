@@ -196,7 +196,8 @@ ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* 
   c->h = (int)height;
   plan_pair(c->w, c->h, 0, 0, &c->plan);
   const PairPlan& p = c->plan;
-  const size_t arena_bytes = p.arena_elems * sizeof(float);
+  // + 64 KB: the row pass prefetches up to one row past a band's last row
+  const size_t arena_bytes = p.arena_elems * sizeof(float) + 65536;
   const size_t part_bytes = size_t(p.nparts) * kNumSums * sizeof(double);
   const size_t items_bytes = p.items.size() * sizeof(WorkItem);
   c->bytes = arena_bytes + part_bytes + kResultDoubles * sizeof(double) + sizeof(PairDesc) + items_bytes;
@@ -229,7 +230,11 @@ ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* 
   d.arena = c->arena;
   d.partials = c->partials;
   d.result = c->result;
-  if ((e = hipMemcpy(c->d_desc, &d, sizeof(PairDesc), hipMemcpyHostToDevice)) != hipSuccess ||
+  // The padding columns right of every plane row must read as zero (the row
+  // scan takes them as the inputs past the image); no kernel writes them
+  // except the row pass's own outputs, which no later pass reads there.
+  if ((e = hipMemset(c->arena, 0, arena_bytes)) != hipSuccess ||
+      (e = hipMemcpy(c->d_desc, &d, sizeof(PairDesc), hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(c->d_items, p.items.data(), items_bytes, hipMemcpyHostToDevice)) != hipSuccess) {
     release(c);
     delete c;

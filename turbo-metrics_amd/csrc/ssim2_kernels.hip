@@ -215,137 +215,184 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 
 // ---------------------------------------------------------------- row pass
 
-// One wave per (scale, channel, quantity, band of kRowsPerWave rows). Lane
-// 3r+k runs cosine term k of row r of the band (cpu.rs:967-1022, row form), so
-// the three terms of a row advance in lock step on neighbouring lanes. Inputs
-// come straight from the XYB planes as float4 (all three lanes of a row read
-// the same address), four columns per group, prefetched four groups ahead
-// into registers; a 16-slot register ring holds the quantity values of
-// columns [n-8, n+8). After each group the three terms are summed in cpu.rs
-// order, (o1 + o3) + o5, through a 1 KB LDS exchange and stored as float4.
-template <int Q>
-__device__ __forceinline__ float row_quantity(float a, float b) {
-  return Q == 0 ? a * a : (Q == 1 ? b * b : (Q == 2 ? a * b : (Q == 3 ? a : b)));
+// One wave per (scale, channel, quantity, band of 64 rows). Lane r scans row r
+// of the band with the three cosine terms of cpu.rs:967-1022 (row form) held in
+// registers, so every lane carries three independent recurrences and forms
+// (o1 + o3) + o5 itself: with about one wave per SIMD the scan is bound by the
+// wave's instruction issue, and this keeps it near 13 instructions per column.
+//
+// Memory is staged through LDS in chunks of 64 rows x 32 columns so that every
+// global access is coalesced (8 rows x 128 B per instruction, instead of one
+// row per lane, which costs the address unit a cache line per lane): buffer
+// loads with scalar offsets fill a padded LDS tile, each lane reads its row
+// back, and the outputs leave the same way. The next chunk's loads are in
+// flight while the current one is scanned.
+//
+// Every plane has at least 8 zero columns of padding (pitch >= w + 8), which
+// supply the zero inputs right of the row; the left ones start as zeros in
+// the ring. The buffer range check turns reads of rows past the image into 0
+// and drops stores outside the band's rows.
+constexpr int kChunk = 32;                      // columns per staged chunk (8 groups of 4)
+constexpr int kTileStride = kChunk + 4;         // floats; b128 reads of one row per lane hit distinct banks
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+constexpr unsigned kRsrcWord3 = 0x00020000u;    // raw buffer, 32-bit data format
+constexpr unsigned kOffNone = 0xC0000000u;      // beyond every plane (< 2^30 floats)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const float* p, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0, (int)bytes, kRsrcWord3);
 }
 
+__device__ __forceinline__ float4 u2f(u32x4 v) {
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+
+// The row pass exchanges data between the lanes of one wave through LDS. A
+// wave's LDS operations execute in issue order, so only the compiler's
+// reordering has to be stopped between the phases.
+__device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory"); }
+
 template <int Q>
-__device__ __forceinline__ void row_scan(gcfloat* __restrict__ rowa, gcfloat* __restrict__ rowb,
-                                         gfloat* __restrict__ out, int w, int pitch, bool ok, int k,
-                                         float4* ks) {
+__device__ __forceinline__ float4 row_quantity(float4 x, float4 y) {
+  if (Q == 0) return make_float4(x.x * x.x, x.y * x.y, x.z * x.z, x.w * x.w);
+  if (Q == 1) return make_float4(y.x * y.x, y.y * y.y, y.z * y.z, y.w * y.w);
+  if (Q == 2) return make_float4(x.x * y.x, x.y * y.y, x.z * y.z, x.w * y.w);
+  if (Q == 3) return x;
+  return y;
+}
+
+// One cpu.rs step (row form) of the three cosine terms at one column.
+struct RowState {
+  float p1 = 0.f, p3 = 0.f, p5 = 0.f, q1 = 0.f, q3 = 0.f, q5 = 0.f;
+  __device__ __forceinline__ float step(float left, float right) {
+    const float sum = left + right;
+    float o1 = sum * kMulIn1, o3 = sum * kMulIn3, o5 = sum * kMulIn5;
+    o1 = fmaf(-1.0f, q1, o1);
+    o3 = fmaf(-1.0f, q3, o3);
+    o5 = fmaf(-1.0f, q5, o5);
+    q1 = p1;
+    q3 = p3;
+    q5 = p5;
+    o1 = fmaf(kMulPrev1, p1, o1);
+    o3 = fmaf(kMulPrev3, p3, o3);
+    o5 = fmaf(kMulPrev5, p5, o5);
+    p1 = o1;
+    p3 = o3;
+    p5 = o5;
+    return (o1 + o3) + o5;
+  }
+};
+
+// Step n (cpu.rs runs n = -4 .. w-1) adds x[n-6] and x[n+4]. Output group g
+// covers n = 4g .. 4g+3: its right inputs are x-group g+1 (columns 4g+4..4g+7)
+// and its left inputs the upper half of x-group g-2 and the lower half of
+// x-group g-1. Chunk k computes output groups 8k..8k+7 (columns 32k..32k+31)
+// from x-groups 8k+1..8k+8, i.e. columns 32k+4..32k+35, staged in LDS; the
+// ring keeps the quantities of the last four x-groups (ring[g & 3] = group g).
+// The buffer range check applies to the per-lane offset only, so rows past the
+// image carry an out-of-range per-lane offset and the column is the scalar one.
+template <int Q>
+__device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buffer_rsrc_t rb,
+                                         __amdgpu_buffer_rsrc_t ro, int w, int pitch, int nrows,
+                                         float4* tile_a, float4* tile_b) {
   constexpr bool NA = Q != 1 && Q != 4;
   constexpr bool NB = Q == 1 || Q == 2 || Q == 4;
+  constexpr int T4 = kTileStride / 4;  // tile row stride in float4
   const int lane = threadIdx.x;
-  const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
-  const float mp = k == 0 ? kMulPrev1 : (k == 1 ? kMulPrev3 : kMulPrev5);
+  const int lr = lane >> 3, ls = lane & 7;  // coalesced layout: row 8i+lr, segment ls
+  const u32x4 z = {0u, 0u, 0u, 0u};
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  // Branch-free loads: the address is clamped into the row (pitch >= 64, so
-  // pitch - 4 is a valid float4 start) and invalid columns or rows are zeroed
-  // by a select, so no exec-mask branch splits the prefetch stream.
-  auto load = [&](int col, float4& a, float4& b) {
-    const int cc = min(col, pitch - 4);
-    a = NA ? to_f4(*(gcfloat4*)(rowa + cc)) : z4;
-    b = NB ? to_f4(*(gcfloat4*)(rowb + cc)) : z4;
-  };
-  // Lanes outside the image see width 0, so one compare per value masks both
-  // the row and the column range (no short-circuit branches).
-  const int wl = ok ? w : 0;
-  auto quant = [&](const float4& a, const float4& b, int col, float* v) {
-    v[0] = col + 0 < wl ? row_quantity<Q>(a.x, b.x) : 0.0f;
-    v[1] = col + 1 < wl ? row_quantity<Q>(a.y, b.y) : 0.0f;
-    v[2] = col + 2 < wl ? row_quantity<Q>(a.z, b.z) : 0.0f;
-    v[3] = col + 3 < wl ? row_quantity<Q>(a.w, b.w) : 0.0f;
-  };
-  float R[16];
+  unsigned voff[8];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) R[i] = 0.0f;
-  {
-    float4 a, b;
-    load(0, a, b);
-    quant(a, b, 0, R);  // columns 0..3 -> slots 0..3
+  for (int i = 0; i < 8; ++i) {
+    const int row = 8 * i + lr;
+    voff[i] = row < nrows ? (unsigned)(row * pitch * 4 + ls * 16) : kOffNone;
   }
-  // Prefetch queue: slot j holds the raw inputs of columns n0+8..n0+11 of the
-  // group that consumes it, loaded kDepth groups earlier.
-  constexpr int kDepth = 8;
-  float4 qa[kDepth], qb[kDepth];
+  const int glast = (w - 1) / 4;  // last output group
+  const int nchunks = glast / 8 + 1;
+  u32x4 qa[8], qb[8];
+  auto load_chunk = [&](int k) {
 #pragma unroll
-  for (int j = 0; j < kDepth; ++j) load(4 + 4 * j, qa[j], qb[j]);
-  float prev = 0.0f, prev2 = 0.0f;
-  // Groups of four outputs cover n = -4 .. w-1; the loop runs whole sets of
-  // kDepth groups so the body is straight-line (groups past the end are masked).
-  // The three-term sum of group g-1 is read back from LDS while group g's
-  // recurrence runs (software pipelining): a wave's LDS operations execute in
-  // issue order, so only compiler reordering has to be prevented.
-  const int ngroups = (w + kRadius - 1 + 3) / 4 + 1;  // one extra group drains the last sum
-  const int nloop = (ngroups + kDepth - 1) / kDepth;
-  const int l1 = (lane + 1) & 63, l2 = (lane + 2) & 63;
-  for (int g0 = 0; g0 < kDepth * nloop; g0 += kDepth) {
-#pragma unroll
-    for (int j = 0; j < kDepth; ++j) {
-      const int n0 = -kRadius + 1 + 4 * (g0 + j);
-      const int base = (12 + 4 * j) & 15;  // n0 mod 16
-      // 1. issue the reads of the previous group's terms
-      float4* kp = ks + ((j + 1) & 1) * 64;  // buffer written by group g-1
-      const float4 x = kp[lane], y = kp[l1], zz = kp[l2];
-      // 2. recurrence of this group
-      float o4[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float sum = R[(base + i - kRadius - 1 + 16) & 15] + R[(base + i + kRadius - 1) & 15];
-        float o = sum * mi;
-        o = fmaf(-1.0f, prev2, o);
-        prev2 = prev;
-        o = fmaf(mp, prev, o);
-        prev = o;
-        o4[i] = o;
-      }
-      // 3. (o1 + o3) + o5 of the previous group, stored by the k = 0 lane
-      const int np = n0 - 4;
-      if (k == 0 && ok && np >= 0 && np < w) {
-        vf4 rv = {(x.x + y.x) + zz.x, (x.y + y.y) + zz.y, (x.z + y.z) + zz.z, (x.w + y.w) + zz.w};
-        *(gfloat4*)(out + np) = rv;
-      }
-      // 4. publish this group's terms
-      asm volatile("" ::: "memory");
-      ks[(j & 1) * 64 + lane] = make_float4(o4[0], o4[1], o4[2], o4[3]);
-      asm volatile("" ::: "memory");
-      // columns n0+8..n0+11 replace n0-8..n0-5 in the ring
-      {
-        float v[4];
-        quant(qa[j], qb[j], n0 + 8, v);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) R[(base + 8 + i) & 15] = v[i];
-      }
-      // Keep the refill of slot j below its consumption: if the scheduler
-      // hoisted the load above it, the slot would need two registers and the
-      // loop back-edge copy would wait for the load (vmcnt(0)).
-      __builtin_amdgcn_sched_barrier(0);
-      load(n0 + 8 + 4 * kDepth, qa[j], qb[j]);
+    for (int i = 0; i < 8; ++i) {
+      qa[i] = NA ? __builtin_amdgcn_raw_buffer_load_b128(ra, voff[i], k * 128 + 16, 0) : z;
+      qb[i] = NB ? __builtin_amdgcn_raw_buffer_load_b128(rb, voff[i], k * 128 + 16, 0) : z;
     }
+  };
+  load_chunk(0);
+  float4 ring[4];
+  {
+    // x-group 0, then the steps n = -4 .. -1 (no outputs): right inputs
+    // x-group 0, left inputs the zeros of x-groups -3 and -2.
+    const unsigned vrow = lane < nrows ? (unsigned)(lane * pitch * 4) : kOffNone;
+    const float4 a = NA ? u2f(__builtin_amdgcn_raw_buffer_load_b128(ra, vrow, 0, 0)) : z4;
+    const float4 b = NB ? u2f(__builtin_amdgcn_raw_buffer_load_b128(rb, vrow, 0, 0)) : z4;
+    ring[0] = row_quantity<Q>(a, b);
+    ring[1] = ring[2] = ring[3] = z4;
+  }
+  RowState st;
+  st.step(0.f, ring[0].x);
+  st.step(0.f, ring[0].y);
+  st.step(0.f, ring[0].z);
+  st.step(0.f, ring[0].w);
+  for (int k = 0; k < nchunks; ++k) {
+    // stage x columns 32k+4 .. 32k+35 of rows 8i+lr into the padded tiles
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (NA) tile_a[(8 * i + lr) * T4 + ls] = u2f(qa[i]);
+      if (NB) tile_b[(8 * i + lr) * T4 + ls] = u2f(qb[i]);
+    }
+    wave_lds_order();
+    load_chunk(k + 1);  // past the last chunk: 0 or columns no output uses
+    float4 out[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float4 xa = NA ? tile_a[lane * T4 + j] : z4;
+      const float4 xb = NB ? tile_b[lane * T4 + j] : z4;
+      ring[(j + 1) & 3] = row_quantity<Q>(xa, xb);  // x-group 8k+j+1
+      const float4 r = ring[(j + 1) & 3], m2 = ring[(j + 2) & 3], m1 = ring[(j + 3) & 3];
+      out[j].x = st.step(m2.z, r.x);
+      out[j].y = st.step(m2.w, r.y);
+      out[j].z = st.step(m1.x, r.z);
+      out[j].w = st.step(m1.y, r.w);
+    }
+    // back through the tile: own row in, coalesced rows out; segments past
+    // the last output group are dropped with an out-of-range offset
+    wave_lds_order();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) tile_a[lane * T4 + j] = out[j];
+    wave_lds_order();
+    const bool seg_ok = 8 * k + ls <= glast;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const float4 v = tile_a[(8 * i + lr) * T4 + ls];
+      const u32x4 u = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+      __builtin_amdgcn_raw_buffer_store_b128(u, ro, seg_ok ? voff[i] : kOffNone, k * 128, 0);
+    }
+    wave_lds_order();
   }
 }
 
 template <int SIDES>
 __global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
                                              const WorkItem* __restrict__ items) {
-  __shared__ float4 ks[2 * 64];
+  __shared__ float4 tiles[2][64 * kTileStride / 4];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b / kNumQ, q = it.b % kNumQ;
   const ScaleDesc& S = P.sc[s];
-  const int lane = threadIdx.x;
-  const int r = lane / 3, k = lane - 3 * r;
-  const int row = it.c * kRowsPerWave + r;
-  const bool ok = r < kRowsPerWave && row < S.h;
-  const long long roff = (long long)(ok ? row : 0) * S.pitch;
-  gcfloat* rowa = gbl(xyb_plane(P, s, 0, c, SIDES) + roff);
-  gcfloat* rowb = gbl(xyb_plane(P, s, 1, c, SIDES) + roff);
-  gfloat* out = gbl(P.arena + S.hrow + (long long)(c * kNumQ + q) * S.plane + roff);
+  const int row0 = it.c * kRowsPerWave;
+  const long long off = (long long)row0 * S.pitch;
+  const int nrows = min(kRowsPerWave, S.h - row0);
+  const long long bytes = (long long)nrows * S.pitch * 4;
+  const __amdgpu_buffer_rsrc_t ra = buffer_rsrc(xyb_plane(P, s, 0, c, SIDES) + off, bytes);
+  const __amdgpu_buffer_rsrc_t rb = buffer_rsrc(xyb_plane(P, s, 1, c, SIDES) + off, bytes);
+  const __amdgpu_buffer_rsrc_t ro =
+      buffer_rsrc(P.arena + S.hrow + (long long)(c * kNumQ + q) * S.plane + off, bytes);
   switch (q) {
-    case 0: row_scan<0>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
-    case 1: row_scan<1>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
-    case 2: row_scan<2>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
-    case 3: row_scan<3>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
-    default: row_scan<4>(rowa, rowb, out, S.w, S.pitch, ok, k, ks); break;
+    case 0: row_scan<0>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
+    case 1: row_scan<1>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
+    case 2: row_scan<2>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
+    case 3: row_scan<3>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
+    default: row_scan<4>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
   }
 }
 

@@ -62,7 +62,9 @@ inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPla
     ScaleDesc& S = d.sc[s];
     S.w = sw[s];
     S.h = sh[s];
-    S.pitch = (sw[s] + 63) / 64 * 64;
+    // >= 8 zero columns right of every row (the row scan reads them as the
+    // zero inputs past the image), rounded to 256 bytes
+    S.pitch = (sw[s] + 8 + 63) / 64 * 64;
     S.colblocks = (sw[s] + kColBlock - 1) / kColBlock;
     S.plane = (long long)S.pitch * S.h;
     S.xyb = off;

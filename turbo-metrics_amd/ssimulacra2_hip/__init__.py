@@ -19,7 +19,9 @@ from pathlib import Path
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-LIB_PATH = _HERE / "libssim2hip.so"
+# SSIM2HIP_LIBRARY may name another build of the same library (an installed
+# copy, a differently tuned build).
+LIB_PATH = Path(os.environ.get("SSIM2HIP_LIBRARY") or (_HERE / "libssim2hip.so"))
 
 __all__ = [
     "Ssimulacra2",
