@@ -342,12 +342,19 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
     }
     wave_lds_order();
     load_chunk(k + 1);  // past the last chunk: 0 or columns no output uses
+    // all tile reads in flight at once (LDS returns them in order), so the
+    // scan waits for each only as it reaches it
+    float4 xa[8], xb[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      xa[j] = NA ? tile_a[lane * T4 + j] : z4;
+      xb[j] = NB ? tile_b[lane * T4 + j] : z4;
+    }
+    __builtin_amdgcn_sched_barrier(0);
     float4 out[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float4 xa = NA ? tile_a[lane * T4 + j] : z4;
-      const float4 xb = NB ? tile_b[lane * T4 + j] : z4;
-      ring[(j + 1) & 3] = row_quantity<Q>(xa, xb);  // x-group 8k+j+1
+      ring[(j + 1) & 3] = row_quantity<Q>(xa[j], xb[j]);  // x-group 8k+j+1
       const float4 r = ring[(j + 1) & 3], m2 = ring[(j + 2) & 3], m1 = ring[(j + 3) & 3];
       out[j].x = st.step(m2.z, r.x);
       out[j].y = st.step(m2.w, r.y);
@@ -361,10 +368,14 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
     for (int j = 0; j < 8; ++j) tile_a[lane * T4 + j] = out[j];
     wave_lds_order();
     const bool seg_ok = 8 * k + ls <= glast;
+    float4 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = tile_a[(8 * i + lr) * T4 + ls];
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const float4 v = tile_a[(8 * i + lr) * T4 + ls];
-      const u32x4 u = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+      const u32x4 u = {__float_as_uint(v[i].x), __float_as_uint(v[i].y), __float_as_uint(v[i].z),
+                       __float_as_uint(v[i].w)};
       __builtin_amdgcn_raw_buffer_store_b128(u, ro, seg_ok ? voff[i] : kOffNone, k * 128, 0);
     }
     wave_lds_order();
@@ -372,7 +383,7 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
 }
 
 template <int SIDES>
-__global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_rows(const PairDesc* __restrict__ pairs,
                                              const WorkItem* __restrict__ items) {
   __shared__ float4 tiles[2][64 * kTileStride / 4];
   const WorkItem it = items[blockIdx.x];
