@@ -21,8 +21,9 @@ constexpr int kSumsPerPair = kMaxScales * 3 * kNumSums;  // 108
 // Tile / block geometry (compile-time; the launch builders use the same values).
 constexpr int kPyrTile = 32;      // scale-0 tile edge of the pyramid kernel
 constexpr int kRowsPerWave = 64;  // rows per row-pass wave (one lane per row)
-constexpr int kColBlock = 16;     // columns per column-pass block
-constexpr int kColChunk = 16;     // rows per column-pass chunk (one pixel per thread)
+constexpr int kColBlock = 32;     // columns per column-pass block
+constexpr int kColChunk = 24;     // rows per column-pass chunk
+constexpr int kPadRows = 4;       // zero rows under every exact-path plane (blur radius - 1)
 constexpr int kThreads = 256;
 
 enum Storage : int {

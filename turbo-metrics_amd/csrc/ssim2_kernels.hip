@@ -414,119 +414,152 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One block per (scale, channel, strip of kColBlock columns). Thread t scans
-// column t%16 for (quantity, cosine term) t/16 (cpu.rs:1054-1115, column form)
-// in chunks of kColChunk rows: the bottom inputs of a chunk are prefetched
-// into registers one chunk ahead, and the top inputs (10 rows up) are the
-// bottom inputs of earlier steps, so each step costs one load. Each chunk's
-// terms go to a double-buffered LDS tile; after one barrier every thread
-// takes one pixel of the chunk, forms (o1+o3)+o5 per quantity and evaluates
-// the SSIM and edge-difference maps with f64 L1/L4 sums (cpu.rs:581-683).
+// One block per (scale, channel, strip of 32 columns); each wave scans two of
+// the blurred quantities (lanes 0-31 one, lanes 32-63 the next), so a block
+// has ceil(NQ / 2) waves, at most one per SIMD. Lane x of quantity slot j
+// scans column x with the three cosine terms of cpu.rs:1054-1115 (column
+// form) in registers and forms (o1 + o3) + o5 itself. Rows go in chunks of
+// kColChunk: the bottom inputs (row n+4 for step n) of the next chunk are
+// loaded while the current one is scanned, and the top inputs (row n-6) are
+// earlier bottom inputs kept in registers. Loads are buffer loads with the row
+// as a scalar offset, clamped to the 4 zero rows under every plane, so no
+// lane computes a row address or a mask. Each chunk's outputs go to a
+// double-buffered LDS tile; after one barrier all threads evaluate the SSIM
+// and edge-difference maps of the chunk's pixels with f64 L1/L4 sums
+// (cpu.rs:581-683).
 template <int SIDES>
-__global__ __launch_bounds__(kThreads) void k_cols(const PairDesc* __restrict__ pairs,
-                                                   const WorkItem* __restrict__ items) {
+__global__ __launch_bounds__(64 * ((SideTraits<SIDES>::NQ + 1) / 2)) void k_cols(
+    const PairDesc* __restrict__ pairs, const WorkItem* __restrict__ items) {
   constexpr int NQ = SideTraits<SIDES>::NQ;
-  constexpr int NC = NQ * 3;
-  constexpr int CB = kColBlock, RY = kColChunk;
-  static_assert(CB * RY == kThreads, "one pixel per thread per chunk");
+  constexpr int NW = (NQ + 1) / 2;
+  constexpr int RY = kColChunk, CB = kColBlock, NT = 64 * NW;
+  static_assert(CB == 32, "two quantities of 32 columns per wave");
   static_assert(RY >= 2 * kRadius, "top inputs come from the previous chunk");
-  __shared__ float O[2][NC][RY][CB];
-  __shared__ double red[kThreads / 64][kNumSums];
+  __shared__ float O[2][NQ][RY][CB];
+  __shared__ double red[NW][kNumSums];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b, xb = it.c * CB;
   const ScaleDesc& S = P.sc[s];
   const int w = S.w, h = S.h, pitch = S.pitch;
-  const int t = threadIdx.x;
-  const int j = t / CB, x = t % CB;
-  const bool scan = j < NC;
-  const int qi = scan ? j / 3 : 0, k = j % 3;
-  const int col = xb + x;
-  const bool colok = col < w;
-  gcfloat* hq = gbl(P.arena + S.hrow + (long long)(c * kNumQ + SideTraits<SIDES>::q(qi)) * S.plane +
-                    min(col, w - 1));
-  const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
-  const float vmp = k == 0 ? kVertMulPrev1 : (k == 1 ? kVertMulPrev3 : kVertMulPrev5);
-  const bool ld = scan && colok;
-  const unsigned hl = ld ? (unsigned)h : 0u;  // rows outside [0, hl) load as zero
-  // maps-phase pixel of this thread within a chunk
-  const int mi_row = t / CB, mx = t % CB, mcol = xb + mx;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int x = lane & 31;
+  const int slot = min(2 * wv + (lane >> 5), NQ - 1);  // lanes past NQ rescan the last one, unused
+  const int q0 = SideTraits<SIDES>::q(2 * wv), q = SideTraits<SIDES>::q(slot);
+  // the wave's first plane as the buffer base; the second one by its distance
+  // (at most 3 planes, < 2^32 bytes); rows are clamped, so no range check is needed
+  const __amdgpu_buffer_rsrc_t rq =
+      __builtin_amdgcn_make_buffer_rsrc(P.arena + S.hrow + (long long)(c * kNumQ + q0) * S.plane, 0, -1, kRsrcWord3);
+  const unsigned vcol = (unsigned)((long long)(q - q0) * S.plane * 4 + (xb + x) * 4);
+  const int row_bytes = pitch * 4;
+  auto load_chunk = [&](int y0, float* B) {
+#pragma unroll
+    for (int i = 0; i < RY; ++i) {
+      const int r = min(y0 + i + kRadius - 1, h + kPadRows - 1);  // rows h .. h+3 are zero
+      B[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rq, vcol, r * row_bytes, 0));
+    }
+  };
   gcfloat* srcp = gbl(xyb_plane(P, s, 0, c, SIDES));
   gcfloat* disp = gbl(xyb_plane(P, s, 1, c, SIDES));
   gfloat* cache_s11 = gbl(P.arena + S.cache + (long long)(3 + c) * S.plane);
   gfloat* cache_mu1 = gbl(P.arena + S.cache + (long long)(6 + c) * S.plane);
 
-  // Branch-free: loads from a clamped row, zeroed by a select outside [0, h).
-  auto load_chunk = [&](int y0, float* B) {
+  // Map inputs (source and distorted XYB, and the cached reference statistics
+  // when only the distorted side runs) of this thread's pixels, loaded one
+  // chunk ahead: the maps phase would otherwise wait out an HBM latency per chunk.
+  constexpr int NM = (RY * CB + NT - 1) / NT;  // map pixels per thread per chunk
+  constexpr int NV = SIDES == kSidesDis ? 4 : (SIDES == kSidesBoth ? 2 : 0);
+  float mv[NM][NV > 0 ? NV : 1], mv_next[NM][NV > 0 ? NV : 1];
+  auto load_maps = [&](int y0c, float (*dst)[NV > 0 ? NV : 1]) {
 #pragma unroll
-    for (int i = 0; i < RY; ++i) {
-      const int rb = y0 + i + kRadius - 1;
-      const int rc = min(max(rb, 0), h - 1);
-      const float v = hq[(long long)rc * pitch];
-      B[i] = (unsigned)rb < hl ? v : 0.0f;
+    for (int m = 0; m < NM; ++m) {
+      const int px = tid + NT * m;
+      const int n = min(max(y0c + px / CB, 0), h - 1), col = min(xb + px % CB, w - 1);
+      const long long off = (long long)n * pitch + col;
+      if (NV >= 2) {
+        dst[m][0] = srcp[off];
+        dst[m][1] = disp[off];
+      }
+      if (NV == 4) {
+        dst[m][2] = cache_s11[off];
+        dst[m][3] = cache_mu1[off];
+      }
     }
   };
+
   float Bprev[RY], Bcur[RY], Bnext[RY];
 #pragma unroll
   for (int i = 0; i < RY; ++i) Bprev[i] = 0.0f;
   int y0 = -kRadius + 1;
   load_chunk(y0, Bcur);
-  float prev = 0.0f, prev2 = 0.0f;
+  load_maps(y0, mv);
+  float p1 = 0.f, p3 = 0.f, p5 = 0.f, q1 = 0.f, q3 = 0.f, q5 = 0.f;
   double acc[kNumSums] = {0, 0, 0, 0, 0, 0};
   for (int ci = 0; y0 < h; ++ci, y0 += RY) {
     load_chunk(y0 + RY, Bnext);
     const int buf = ci & 1;
-    if (scan) {
+    load_maps(y0 + RY, mv_next);
 #pragma unroll
-      for (int i = 0; i < RY; ++i) {
-        // top input row y0+i-6 = bottom input of step i-10
-        const float top = i >= 2 * kRadius ? Bcur[i - 2 * kRadius] : Bprev[i + RY - 2 * kRadius];
-        const float sum = top + Bcur[i];
-        const float tt = fmaf(prev, vmp, prev2);
-        const float o = fmaf(sum, mi, -tt);
-        prev2 = prev;
-        prev = o;
-        O[buf][j][i][x] = o;
-      }
+    for (int i = 0; i < RY; ++i) {
+      // top input row y0+i-6 = bottom input of step i-10
+      const float top = i >= 2 * kRadius ? Bcur[i - 2 * kRadius] : Bprev[i + RY - 2 * kRadius];
+      const float sum = top + Bcur[i];
+      const float t1 = fmaf(p1, kVertMulPrev1, q1), t3 = fmaf(p3, kVertMulPrev3, q3),
+                  t5 = fmaf(p5, kVertMulPrev5, q5);
+      const float o1 = fmaf(sum, kMulIn1, -t1), o3 = fmaf(sum, kMulIn3, -t3), o5 = fmaf(sum, kMulIn5, -t5);
+      q1 = p1;
+      q3 = p3;
+      q5 = p5;
+      p1 = o1;
+      p3 = o3;
+      p5 = o5;
+      O[buf][slot][i][x] = (o1 + o3) + o5;
     }
     lds_barrier();
-    const int n = y0 + mi_row;
-    if (n >= 0 && n < h && mcol < w) {
-      float bq[NQ];
 #pragma unroll
-      for (int q2 = 0; q2 < NQ; ++q2)
-        bq[q2] = (O[buf][3 * q2][mi_row][mx] + O[buf][3 * q2 + 1][mi_row][mx]) + O[buf][3 * q2 + 2][mi_row][mx];
-      const long long off = (long long)n * pitch + mcol;
-      if (SIDES == kSidesRef) {
-        cache_s11[off] = bq[0];
-        cache_mu1[off] = bq[1];
-      } else {
-        float s11, s22, s12, mu1, mu2;
-        if (SIDES == kSidesBoth) {
-          s11 = bq[0];
-          s22 = bq[1];
-          s12 = bq[2];
-          mu1 = bq[3];
-          mu2 = bq[4];
-        } else {
-          s11 = cache_s11[off];
-          mu1 = cache_mu1[off];
-          s22 = bq[0];
-          s12 = bq[1];
-          mu2 = bq[2];
+    for (int m = 0; m < (RY * CB + NT - 1) / NT; ++m) {
+      const int px = tid + NT * m;
+      const int i = px / CB, col = xb + px % CB, n = y0 + i;
+      if ((RY * CB) % NT == 0 || px < RY * CB) {
+        if (n >= 0 && n < h && col < w) {
+          float bq[NQ];
+#pragma unroll
+          for (int q2 = 0; q2 < NQ; ++q2) bq[q2] = O[buf][q2][i][px % CB];
+          const long long off = (long long)n * pitch + col;
+          if (SIDES == kSidesRef) {
+            cache_s11[off] = bq[0];
+            cache_mu1[off] = bq[1];
+          } else {
+            const float* v = mv[m];
+            float s11, s22, s12, mu1, mu2;
+            if (SIDES == kSidesBoth) {
+              s11 = bq[0];
+              s22 = bq[1];
+              s12 = bq[2];
+              mu1 = bq[3];
+              mu2 = bq[4];
+            } else {
+              s11 = v[2];
+              mu1 = v[3];
+              s22 = bq[0];
+              s12 = bq[1];
+              mu2 = bq[2];
+            }
+            const double d = ssim_error(mu1, mu2, s11, s22, s12);
+            const double d2 = d * d;
+            acc[0] += d;
+            acc[1] += d2 * d2;
+            const double d1 = edge_ratio(v[0], mu1, v[1], mu2);
+            const double art = d1 > 0.0 ? d1 : 0.0;
+            const double det = -d1 > 0.0 ? -d1 : 0.0;
+            const double a2 = art * art, e2 = det * det;
+            acc[2] += art;
+            acc[3] += a2 * a2;
+            acc[4] += det;
+            acc[5] += e2 * e2;
+          }
         }
-        const double d = ssim_error(mu1, mu2, s11, s22, s12);
-        const double d2 = d * d;
-        acc[0] += d;
-        acc[1] += d2 * d2;
-        const double d1 = edge_ratio(srcp[off], mu1, disp[off], mu2);
-        const double art = d1 > 0.0 ? d1 : 0.0;
-        const double det = -d1 > 0.0 ? -d1 : 0.0;
-        const double a2 = art * art, e2 = det * det;
-        acc[2] += art;
-        acc[3] += a2 * a2;
-        acc[4] += det;
-        acc[5] += e2 * e2;
       }
     }
 #pragma unroll
@@ -534,20 +567,23 @@ __global__ __launch_bounds__(kThreads) void k_cols(const PairDesc* __restrict__ 
       Bprev[i] = Bcur[i];
       Bcur[i] = Bnext[i];
     }
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+#pragma unroll
+      for (int k = 0; k < (NV > 0 ? NV : 1); ++k) mv[m][k] = mv_next[m][k];
   }
   if (SIDES == kSidesRef) return;
-  // Block reduction in a fixed order: wave shuffles, then waves 0..3 in sequence.
-  const int lane = t & 63, wv = t >> 6;
+  // Block reduction in a fixed order: wave shuffles, then waves 0..NW-1 in sequence.
 #pragma unroll
-  for (int q = 0; q < kNumSums; ++q) {
-    const double v = wave_sum(acc[q]);
-    if (lane == 0) red[wv][q] = v;
+  for (int k = 0; k < kNumSums; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[wv][k] = v;
   }
   lds_barrier();
-  if (t < kNumSums) {
+  if (tid < kNumSums) {
     double v = 0.0;
-    for (int q = 0; q < kThreads / 64; ++q) v += red[q][t];
-    P.partials[(long long)(S.part + c * S.colblocks + it.c) * kNumSums + t] = v;
+    for (int k = 0; k < NW; ++k) v += red[k][tid];
+    P.partials[(long long)(S.part + c * S.colblocks + it.c) * kNumSums + tid] = v;
   }
 }
 
@@ -617,7 +653,15 @@ hipError_t launch_rows(const PairDesc* pairs, const WorkItem* items, int nitems,
 hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems, int sides,
                        hipStream_t stream) {
   if (nitems <= 0) return hipSuccess;
-  SSIM2_DISPATCH_SIDES(k_cols, dim3(nitems), dim3(kThreads), stream, pairs, items);
+  if (sides == kSidesBoth)
+    hipLaunchKernelGGL(k_cols<kSidesBoth>, dim3(nitems), dim3(64 * ((SideTraits<kSidesBoth>::NQ + 1) / 2)), 0, stream, pairs,
+                       items);
+  else if (sides == kSidesRef)
+    hipLaunchKernelGGL(k_cols<kSidesRef>, dim3(nitems), dim3(64 * ((SideTraits<kSidesRef>::NQ + 1) / 2)), 0, stream, pairs,
+                       items);
+  else
+    hipLaunchKernelGGL(k_cols<kSidesDis>, dim3(nitems), dim3(64 * ((SideTraits<kSidesDis>::NQ + 1) / 2)), 0, stream, pairs,
+                       items);
   return hipGetLastError();
 }
 

@@ -66,7 +66,9 @@ inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPla
     // zero inputs past the image), rounded to 256 bytes
     S.pitch = (sw[s] + 8 + 63) / 64 * 64;
     S.colblocks = (sw[s] + kColBlock - 1) / kColBlock;
-    S.plane = (long long)S.pitch * S.h;
+    // + 4 zero rows under every plane: the column pass reads rows h .. h+3 as
+    // the zero inputs below the image
+    S.plane = (long long)S.pitch * (S.h + kPadRows);
     S.xyb = off;
     S.hrow = off + 6 * S.plane;
     S.cache = off + (6 + 3 * kNumQ) * S.plane;
