@@ -10,10 +10,13 @@ typedef __attribute__((address_space(1))) const unsigned char gin_u8;
 typedef __attribute__((address_space(1))) const unsigned short gin_u16;
 typedef __attribute__((address_space(1))) const float gin_f32;
 
-// Linear RGB of pixel (gx, gy). `lut` is the sRGB u8 table of the calling
-// translation unit. Chroma of 4:2:0 inputs is taken from the sample of the
-// pixel's 2x2 quad (nearest), as crates/cuda-colorspace-kernel/src/biplanar.rs:31-35.
-__device__ __forceinline__ void read_linear(const InputView& v, int gx, int gy, const float* lut, float* rgb) {
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+
+// Linear RGB of pixel (gx, gy). `lut` is the sRGB u8 table (constant memory or
+// an LDS copy). Chroma of 4:2:0 inputs is taken from the sample of the pixel's
+// 2x2 quad (nearest), as crates/cuda-colorspace-kernel/src/biplanar.rs:31-35.
+template <typename Lut>
+__device__ __forceinline__ void read_linear(const InputView& v, int gx, int gy, Lut lut, float* rgb) {
   switch (v.kind) {
     case kInputSrgbU8: {
       gin_u8* px = (gin_u8*)v.p + gy * v.pitch + gx * 3;

@@ -135,6 +135,15 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
     in.kind = P.in_kind;
   }
 
+  // the sRGB table in LDS: the lookups are divergent, so constant memory would
+  // serve them as one global load each
+  __shared__ float lut_s[256];
+  if (in.kind == kInputSrgbU8) {
+    for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut[i];
+    lds_barrier();
+  }
+  lds_cfloat* lut = (lds_cfloat*)lut_s;
+
   // Scale 0: input -> linear (LDS) -> XYB planes.
   for (int p = tid; p < T0 * T0; p += kThreads) {
     const int lx = p % T0, ly = p / T0;
@@ -143,7 +152,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
     for (int im = 0; im < NI; ++im) {
       const int img = SIDES == kSidesDis ? 1 : im;
       float rgb[3];
-      read_linear(input_of(in, img), gx, gy, c_lut, rgb);
+      read_linear(input_of(in, img), gx, gy, lut, rgb);
       if (in.kind != kInputLinearF32) {
         float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
         if (lo) {

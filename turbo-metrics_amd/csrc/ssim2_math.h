@@ -52,15 +52,21 @@ SSIM2_HD float cbrtf_ref(float x) {
   const double kCbrt2 = 1.2599210498948731648;     // 2^(1/3)
   const double kSqrCbrt2 = 1.5874010519681994748;  // 2^(2/3)
   int e;
-  float m = frexpf(fabsf(x), &e);
-  if (e == 0 && (x == 0.0f || !(x == x) || isinf(x))) return x + x;
-  float u = (float)(0.492659620528969547 +
-                    (0.697570460207922770 - 0.191502161678719066 * (double)m) * (double)m);
-  float t2 = u * u * u;
-  int r = e % 3;  // in [-2, 2]
-  double f = r == 0 ? 1.0 : (r == 1 ? kCbrt2 : (r == 2 ? kSqrCbrt2 : (r == -1 ? 1.0 / kCbrt2 : 1.0 / kSqrCbrt2)));
-  float y = (float)((double)u * ((double)t2 + 2.0 * (double)m) / (2.0 * (double)t2 + (double)m) * f);
-  return ldexpf(x > 0.0f ? y : -y, e / 3);
+  const float m = frexpf(fabsf(x), &e);
+  const float u = (float)(0.492659620528969547 +
+                          (0.697570460207922770 - 0.191502161678719066 * (double)m) * (double)m);
+  const float t2 = u * u * u;
+  const int r = e % 3;  // in [-2, 2]
+  // the method's five factors 2^(r/3), as doubles; flat selects, not branches
+  double f = 1.0;
+  f = r == 1 ? kCbrt2 : f;
+  f = r == 2 ? kSqrCbrt2 : f;
+  f = r == -1 ? 1.0 / kCbrt2 : f;
+  f = r == -2 ? 1.0 / kSqrCbrt2 : f;
+  const float y = (float)((double)u * ((double)t2 + 2.0 * (double)m) / (2.0 * (double)t2 + (double)m) * f);
+  const float res = ldexpf(x > 0.0f ? y : -y, e / 3);
+  // zero, infinity and NaN: x + x (frexpf leaves them with exponent 0)
+  return (x == 0.0f || !(x == x) || isinf(x)) ? x + x : res;
 }
 
 // cpu.rs:458-496 px_linear_rgb_to_xyb followed by the positive shift.
