@@ -130,7 +130,7 @@ void release(ssim2_batch* b) {
 
 ssim2_status enqueue_kernels(ssim2_batch* b, hipStream_t s, int cfg, int mult) {
   const int* meta = meta_ptr(b->d_index);
-  BHIP(launch_fast_pyramid(b->storage, b->d_desc, pyr_prefix(b, b->d_index), meta, b->cus * 2, s));
+  BHIP(launch_fast_pyramid(b->storage, b->d_desc, pyr_prefix(b, b->d_index), meta, b->cus * 4, s));
   BHIP(launch_fast_tile(b->storage, cfg, b->d_desc, tile_prefix(b, b->d_index), meta, b->cus * mult, s));
   BHIP(launch_fast_finalize(b->d_desc, meta, b->cus * 4, (int)b->max_pairs, s));
   return SSIM2_OK;

@@ -133,6 +133,28 @@ __device__ __forceinline__ int find_pair(const int* __restrict__ prefix, int npa
   return lo;
 }
 
+// A contiguous range of the flat item list per block (equal-work items), and
+// the owning pair tracked by a cursor that only moves forward: one prefix load
+// per pair boundary instead of a binary search per item.
+struct ItemRange {
+  const int* prefix;
+  int first, end, pair, next;
+  __device__ ItemRange(const int* __restrict__ pre, int npairs, int n) : prefix(pre) {
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    first = min(n, (int)blockIdx.x * per);
+    end = min(n, first + per);
+    pair = first < n ? find_pair(pre, npairs, first) : 0;
+    next = first < n ? pre[pair + 1] : 0;
+  }
+  __device__ int pair_of(int idx) {
+    while (idx >= next) {
+      ++pair;
+      next = prefix[pair + 1];
+    }
+    return pair;
+  }
+};
+
 __device__ __forceinline__ double wave_sum_d(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
@@ -149,10 +171,14 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
   constexpr int T0 = kPyrTile;
   __shared__ float bufA[6][T0][T0 + 1];
   __shared__ float bufB[6][T0 / 2][T0 / 2 + 1];
+  __shared__ float lut_s[256];  // divergent lookups: LDS, not constant memory
   const int npairs = meta[0], n = prefix[npairs];
   const int tid = threadIdx.x;
-  for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
-    const int pi = find_pair(prefix, npairs, idx);
+  for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut_fast[i];
+  lds_cfloat* lut = (lds_cfloat*)lut_s;
+  ItemRange items(prefix, npairs, n);
+  for (int idx = items.first; idx < items.end; ++idx) {
+    const int pi = items.pair_of(idx);
     const PairDesc& P = pairs[pi];
     const int local = idx - prefix[pi];
     const int ntx = (P.w + T0 - 1) / T0;
@@ -165,7 +191,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
 #pragma unroll
       for (int img = 0; img < 2; ++img) {
         float rgb[3];
-        read_linear(input_of(P, img), gx, gy, c_lut_fast, rgb);
+        read_linear(input_of(P, img), gx, gy, lut, rgb);
 #pragma unroll
         for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = rgb[c];
       }
@@ -224,8 +250,9 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
   __shared__ double red[kThreads / 64][18];
   const int npairs = meta[0], n = prefix[npairs];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
-    const int pi = find_pair(prefix, npairs, idx);
+  ItemRange items(prefix, npairs, n);
+  for (int idx = items.first; idx < items.end; ++idx) {
+    const int pi = items.pair_of(idx);
     const PairDesc& P = pairs[pi];
     int s = 0, tile = idx - prefix[pi];
     while (s + 1 < P.nscales && tile >= P.sc[s + 1].tile0) ++s;
