@@ -13,7 +13,8 @@
 //        x - mu = (x' - blur(x')) + c*e,  mu1 - mu2 = blur(r') - blur(d')
 //    which removes most f32 cancellation in sigma - mu^2;
 //  * the cube root is exp2(log2(x)/3) plus one Newton step (f32), the error
-//    maps are f32 (L1/L4 sums per block in f64);
+//    maps are f32 with hardware reciprocals for their two quotients, tile sums
+//    are f32 within a wave and f64 across waves and tiles;
 //  * XYB of scales >= 1 is stored in HBM as (value - shift) in the storage type
 //    (f32, IEEE half or bfloat16); scale 0 is recomputed from the u8 input
 //    inside each tile.
@@ -155,8 +156,16 @@ struct ItemRange {
   }
 };
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+// Sum over the 64 lanes of a wave in a fixed order with DPP adds (no LDS
+// round trips): butterflies within quads and rows of 16, then the row totals
+// are chained into lane 63, which returns the total.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v += __builtin_amdgcn_update_dpp(0.0f, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x124, 0xF, 0xF, false);  // row_ror:4
+  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
   return v;
 }
 
@@ -365,12 +374,16 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
           const float num_m = fmaf(md, -md, 1.0f);
           const float num_s = fmaf(2.0f, cov, kC2);
           const float den = (var1 + var2) + kC2;
-          const float dd = fmaxf(1.0f - (num_m * num_s) / den, 0.0f);
+          // quotients by hardware reciprocal; equal operands (identical images)
+          // keep their exact quotient 1, so identical pairs still score 100
+          const float nm = num_m * num_s;
+          const float dd = fmaxf(1.0f - (nm == den ? 1.0f : nm * __builtin_amdgcn_rcpf(den)), 0.0f);
           const float d2 = dd * dd;
           acc[c][0] += dd;
           acc[c][1] += d2 * d2;
           const float src = xs[0][c][oy0 + j + 4][ox + 4], dis = xs[1][c][oy0 + j + 4][ox + 4];
-          const float d1 = (1.0f + fabsf((dis - m2) + ec)) / (1.0f + fabsf((src - m1) + ec)) - 1.0f;
+          const float en = 1.0f + fabsf((dis - m2) + ec), ed = 1.0f + fabsf((src - m1) + ec);
+          const float d1 = en == ed ? 0.0f : en * __builtin_amdgcn_rcpf(ed) - 1.0f;
           const float art = fmaxf(d1, 0.0f), det = fmaxf(-d1, 0.0f);
           const float a2 = art * art, e2 = det * det;
           acc[c][2] += art;
@@ -380,13 +393,14 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
         }
       }
     }
-    // Block sums in f64, fixed order: wave shuffles, then waves in sequence.
+    // Block sums in a fixed order: f32 within each wave (DPP), then f64 over
+    // the waves in sequence.
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        const double v = wave_sum_d((double)acc[c][j]);
-        if (lane == 0) red[wv][c * 6 + j] = v;
+        const float v = wave_sum_dpp(acc[c][j]);
+        if (lane == 63) red[wv][c * 6 + j] = (double)v;
       }
     fast_barrier();
     if (tid < 18) {
