@@ -85,7 +85,7 @@ bool plan_fast(int w, int h, int storage, int cfg, size_t arena_off, int part_ba
 struct ssim2_batch {
   int storage = kStorageF32;
   int tile_cfg = 0;
-  int grid_mult = 4;
+  int grid_mult = 8;
   uint32_t max_pairs = 0;
   uint64_t max_pixels = 0;
   size_t arena_cap = 0, part_cap = 0;
@@ -436,7 +436,7 @@ ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint3
   hipEvent_t e0, e1;
   BHIP(hipEventCreate(&e0));
   BHIP(hipEventCreate(&e1));
-  static const int kMults[] = {2, 4, 8};
+  static const int kMults[] = {4, 8, 16};
   float best = 1e30f;
   int best_cfg = b->tile_cfg, best_mult = b->grid_mult;
   std::string rep = "tile     grid_x_cu  ms/batch\n";

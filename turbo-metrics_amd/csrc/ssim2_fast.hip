@@ -248,11 +248,12 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
 
 // ---------------------------------------------------------------- fused tile
 
-template <typename T, int TX, int TY>
+// RU = output rows per vertical-pass unit (TY / RU units per column).
+template <typename T, int TX, int TY, int RU>
 __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restrict__ pairs,
                                                         const int* __restrict__ prefix,
                                                         const int* __restrict__ meta) {
-  static_assert(TX % 4 == 0 && TY % 2 == 0, "tile shape");
+  static_assert(TX % 4 == 0 && TY % RU == 0, "tile shape");
   constexpr int IW = TX + 8, IH = TY + 8;
   __shared__ float xs[2][3][IH][IW + 1];  // shifted XYB, zero outside the image
   __shared__ float hq[5][IH][TX + 1];     // horizontal pass, one channel at a time
@@ -338,18 +339,18 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
         }
       }
       fast_barrier();
-      // Phase 3: vertical FIR (two rows per unit), statistics, error maps.
+      // Phase 3: vertical FIR (RU rows per unit), statistics, error maps.
       const float sh = xyb_shift(c);
-      for (int e = tid; e < (TY / 2) * TX; e += kThreads) {
-        const int oy0 = (e / TX) * 2, ox = e % TX;
-        float bq[5][2];
+      for (int e = tid; e < (TY / RU) * TX; e += kThreads) {
+        const int oy0 = (e / TX) * RU, ox = e % TX;
+        float bq[5][RU];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-          float col[10];
+          float col[RU + 8];
 #pragma unroll
-          for (int k = 0; k < 10; ++k) col[k] = hq[q][oy0 + k][ox];
+          for (int k = 0; k < RU + 8; ++k) col[k] = hq[q][oy0 + k][ox];
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
+          for (int j = 0; j < RU; ++j) {
             float r = kH4 * (col[j] + col[j + 8]);
             r = fmaf(kH3, col[j + 1] + col[j + 7], r);
             r = fmaf(kH2, col[j + 2] + col[j + 6], r);
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
         const int gx = x0 + ox;
         const float b1x = tap_sum(gx, w);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < RU; ++j) {
           const int gy = y0 + oy0 + j;
           if (gx >= w || gy >= h) continue;
           const float b1 = b1x * tap_sum(gy, h);
@@ -435,16 +436,16 @@ static hipError_t launch_tile_t(int tile_cfg, const PairDesc* pairs, const int* 
                                 int grid, hipStream_t stream) {
   switch (tile_cfg) {
     case 0:
-      hipLaunchKernelGGL((k_fast_tile<T, 32, 16>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      hipLaunchKernelGGL((k_fast_tile<T, 32, 16, 2>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
     case 1:
-      hipLaunchKernelGGL((k_fast_tile<T, 64, 16>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      hipLaunchKernelGGL((k_fast_tile<T, 64, 16, 2>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
     case 2:
-      hipLaunchKernelGGL((k_fast_tile<T, 32, 32>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      hipLaunchKernelGGL((k_fast_tile<T, 32, 32, 2>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
     default:
-      hipLaunchKernelGGL((k_fast_tile<T, 16, 16>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      hipLaunchKernelGGL((k_fast_tile<T, 32, 24, 3>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
   }
   return hipGetLastError();
