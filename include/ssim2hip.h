@@ -296,6 +296,59 @@ ssim2_status ssim2_server_poll(ssim2_server* server, ssim2_result* out, uint32_t
 ssim2_status ssim2_server_drain(ssim2_server* server, double timeout_s);
 ssim2_status ssim2_server_stats(ssim2_server* server, ssim2_server_stats_t* stats);
 
+/* ---- video driver and statistics ------------------------------------------ */
+
+/* quick_stats::full::Stats (crates/quick-stats/src/lib.rs:1-52): population and
+ * sample variance, percentiles by linear interpolation between ranks. */
+typedef struct {
+  double min, max, mean;
+  double var, sample_var, stddev, sample_stddev;
+  double p1, p5, p50, p95, p99;
+} ssim2_stats;
+
+/* Stats::compute (quick-stats lib.rs:21-52) over n >= 1 values. */
+ssim2_status ssim2_stats_compute(const double* values, size_t n, ssim2_stats* out);
+
+/* One decoded frame in device memory (HwFrame, crates/turbo-metrics/src/lib.rs:128-133):
+ * packed sRGB u8 or YUV 4:2:0. */
+typedef enum { SSIM2_FRAME_SRGB = 0, SSIM2_FRAME_YUV420 = 1 } ssim2_frame_kind;
+typedef struct {
+  uint32_t kind; /* ssim2_frame_kind */
+  uint32_t reserved0;
+  ssim2_image srgb;
+  ssim2_yuv_image yuv;
+} ssim2_frame;
+
+/* FrameSource::next_frame (lib.rs:151-160): fill *frame with the next frame and
+ * return 1, return 0 at the end of the stream, or a negative value on error.
+ * The frame only has to stay valid until the next call on the same source. */
+typedef int (*ssim2_frame_source)(void* user, ssim2_frame* frame);
+
+typedef enum { SSIM2_VIDEO_EXACT = 0, SSIM2_VIDEO_FAST = 1 } ssim2_video_mode;
+
+/* Options (turbo_metrics::Options, lib.rs:44-59) plus the scoring form. */
+typedef struct {
+  uint32_t struct_size; /* sizeof(ssim2_video_options) */
+  uint32_t every;       /* score one decoded pair in `every` (0 or 1: all) */
+  uint32_t skip;        /* leading frames dropped from both sources */
+  uint32_t skip_ref;    /* ... and additionally from the reference source */
+  uint32_t skip_dis;    /* ... and additionally from the distorted source */
+  uint32_t frames;      /* stop after this many decoded pairs (0: all) */
+  uint32_t mode;        /* ssim2_video_mode: exact path pair by pair, or the batched fast form */
+  uint32_t storage;     /* fast form: ssim2_storage */
+  uint32_t batch;       /* fast form: pairs per launch (0: 8) */
+  uint32_t reserved0;
+} ssim2_video_options;
+
+/* TurboMetrics::compute_all for ssimulacra2 (lib.rs:367-442): pulls frame pairs
+ * from the two sources, applies the sampling options, scores the kept pairs and
+ * returns up to `capacity` per-frame scores, their count and their statistics
+ * (MetricAggregate, lib.rs:61-77). */
+ssim2_status ssim2_video_compute_all(uint32_t width, uint32_t height, const ssim2_video_options* options,
+                                     ssim2_frame_source ref_source, void* ref_user, ssim2_frame_source dis_source,
+                                     void* dis_user, ssim2_stream stream, double* scores, size_t capacity,
+                                     size_t* frame_count, ssim2_stats* stats);
+
 /* ---- diagnostics --------------------------------------------------------- */
 
 /* The raw per-scale sums behind the last score: sums[scale*18 + c*6 + j],

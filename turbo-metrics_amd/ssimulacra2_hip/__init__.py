@@ -96,6 +96,23 @@ class _YuvPair(ctypes.Structure):
 YUV_FORMATS = {"yuv420p10": 0, "p016": 1}
 
 
+class _Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("min", "max", "mean", "var", "sample_var", "stddev",
+                                                 "sample_stddev", "p1", "p5", "p50", "p95", "p99")]
+
+
+class _Frame(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("reserved0", ctypes.c_uint32), ("srgb", _Image), ("yuv", _Yuv)]
+
+
+class _VideoOptions(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("struct_size", "every", "skip", "skip_ref", "skip_dis", "frames",
+                                                 "mode", "storage", "batch", "reserved0")]
+
+
+FRAME_SOURCE = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(_Frame))
+
+
 class _BatchOptions(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32),
@@ -192,6 +209,9 @@ def _load():
         "ssim2_yuv_free": (P(_Yuv),),
         "ssim2_yuv_upload": (P(_Yuv), vp, vp, vp),
         "ssim2_batch_compute_yuv_async": (vp, P(_YuvPair), u32, vp),
+        "ssim2_stats_compute": (P(dbl), sz, P(_Stats)),
+        "ssim2_video_compute_all": (u32, u32, P(_VideoOptions), FRAME_SOURCE, vp, FRAME_SOURCE, vp, vp, P(dbl), sz,
+                                    P(sz), P(_Stats)),
         "ssim2_image_alloc": (u32, u32, u32, img),
         "ssim2_image_free": (img,),
         "ssim2_image_upload": (img, vp, sz),
