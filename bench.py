@@ -63,15 +63,20 @@ def scale_dims(w, h):
     return dims
 
 
-def algorithmic_bytes(w, h):
-    """Compulsory HBM bytes per launch of each exact-path kernel for one pair.
-
-    pyramid: 6 B/px of u8 input at scale 0 + 6 XYB planes (f32) written per scale;
-    rows:    6 XYB planes read + 15 row-pass planes written per scale;
-    cols:    15 row-pass planes + 6 XYB planes read per scale (sums only out).
-    """
-    sn = sum(a * b for a, b in scale_dims(w, h))
-    return {"pyramid": 6 * w * h + 24 * sn, "rows": 84 * sn, "cols": 84 * sn, "finalize": 0}
+def algorithmic_bytes(w, h, S=4, I=6):
+    """Algorithmic HBM bytes per launch of each exact-path kernel for one pair,
+    from the per-kernel bytes of the north-star decomposition (S = bytes per
+    stored plane sample, I = input bytes per scale-0 pixel for the pair):
+      ingest (scale 0) I + 12S, downscale 7.5S per scale that has a next one,
+      xyb (scales >= 1) 12S                      -> k_pyramid (all fused)
+      blur A (first pass, products in registers) 21S -> k_rows
+      blur B (second pass) 30S + maps/partials 21S -> k_cols (fused)
+    per pixel of each scale. The fused kernels move less than this (counted
+    bytes are reported beside it)."""
+    dims = scale_dims(w, h)
+    n = [a * b for a, b in dims]
+    pyr = (I + 12 * S) * n[0] + 7.5 * S * sum(n[:-1]) + 12 * S * sum(n[1:])
+    return {"pyramid": int(pyr), "rows": 21 * S * sum(n), "cols": 51 * S * sum(n), "finalize": 0}
 
 
 def load_traffic(kernel):
@@ -474,6 +479,10 @@ def main():
     kms = plain["kms"]
     dom = max(("pyramid", "rows", "cols"), key=lambda k: kms[k])
     achieved = alg[dom] / (kms[dom] * 1e-3) / 1e9
+    per_kernel = {k: {"ms": round(kms[k], 5), "algorithmic_bytes": alg[k],
+                      "achieved_gbs": round(alg[k] / (kms[k] * 1e-3) / 1e9, 1),
+                      "frac": round(alg[k] / (kms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "counted_bytes": load_traffic(k)} for k in ("pyramid", "rows", "cols")}
 
     batch_legs, serving, bhost, bspec, mixed = [], None, None, None, None
     if args.workload == "all":
@@ -540,6 +549,10 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": load_traffic(dom),
             "algorithmic_bytes": alg[dom],
+            "kernels": per_kernel,
+            "pair": {"algorithmic_bytes": sum(alg.values()),
+                     "achieved_gbs": round(sum(alg.values()) * args.steps / plain["dt"] / 1e9, 1),
+                     "frac": round(sum(alg.values()) * args.steps / plain["dt"] / 1e9 / HBM_PEAK_GBS, 4)},
         },
         "cpu_baseline": None,
     }

@@ -20,7 +20,7 @@ constexpr int kSumsPerPair = kMaxScales * 3 * kNumSums;  // 108
 
 // Tile / block geometry (compile-time; the launch builders use the same values).
 constexpr int kPyrTile = 32;      // scale-0 tile edge of the pyramid kernel
-constexpr int kRowsPerWave = 64;  // rows per row-pass wave (one lane per row)
+constexpr int kRowsPerWave = 20;  // rows per row-pass wave (five lane triples per 16-lane row)
 constexpr int kColBlock = 32;     // columns per column-pass block
 constexpr int kColChunk = 24;     // rows per column-pass chunk
 constexpr int kPadRows = 4;       // zero rows under every exact-path plane (blur radius - 1)

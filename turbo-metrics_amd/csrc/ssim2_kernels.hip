@@ -224,18 +224,17 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 
 // ---------------------------------------------------------------- row pass
 
-// One wave per (scale, channel, quantity, band of 64 rows). Lane r scans row r
-// of the band with the three cosine terms of cpu.rs:967-1022 (row form) held in
-// registers, so every lane carries three independent recurrences and forms
-// (o1 + o3) + o5 itself: with about one wave per SIMD the scan is bound by the
-// wave's instruction issue, and this keeps it near 13 instructions per column.
+// One wave per (scale, channel, quantity, band of 20 rows). The scan along a
+// row is sequential (cpu.rs:967-1022, row form), so its time is the number of
+// instructions a wave issues per column: each row is spread over three lanes,
+// one per cosine term, which issue one recurrence each per column and combine
+// the terms with two DPP adds. That leaves more waves than SIMDs at 1080p.
 //
-// Memory is staged through LDS in chunks of 64 rows x 32 columns so that every
+// Memory is staged through LDS in chunks of 20 rows x 32 columns so that every
 // global access is coalesced (8 rows x 128 B per instruction, instead of one
-// row per lane, which costs the address unit a cache line per lane): buffer
-// loads with scalar offsets fill a padded LDS tile, each lane reads its row
-// back, and the outputs leave the same way. The next chunk's loads are in
-// flight while the current one is scanned.
+// row per lane): buffer loads with scalar column offsets fill a padded LDS
+// tile, each lane reads its row back, and the outputs leave the same way. The
+// next chunk's loads are in flight while the current one is scanned.
 //
 // Every plane has at least 8 zero columns of padding (pitch >= w + 8), which
 // supply the zero inputs right of the row; the left ones start as zeros in
@@ -269,61 +268,60 @@ __device__ __forceinline__ float4 row_quantity(float4 x, float4 y) {
   return y;
 }
 
-// One cpu.rs step (row form) of the three cosine terms at one column.
-struct RowState {
-  float p1 = 0.f, p3 = 0.f, p5 = 0.f, q1 = 0.f, q3 = 0.f, q5 = 0.f;
-  __device__ __forceinline__ float step(float left, float right) {
-    const float sum = left + right;
-    float o1 = sum * kMulIn1, o3 = sum * kMulIn3, o5 = sum * kMulIn5;
-    o1 = fmaf(-1.0f, q1, o1);
-    o3 = fmaf(-1.0f, q3, o3);
-    o5 = fmaf(-1.0f, q5, o5);
-    q1 = p1;
-    q3 = p3;
-    q5 = p5;
-    o1 = fmaf(kMulPrev1, p1, o1);
-    o3 = fmaf(kMulPrev3, p3, o3);
-    o5 = fmaf(kMulPrev5, p5, o5);
-    p1 = o1;
-    p3 = o3;
-    p5 = o5;
-    return (o1 + o3) + o5;
-  }
-};
-
 // Step n (cpu.rs runs n = -4 .. w-1) adds x[n-6] and x[n+4]. Output group g
 // covers n = 4g .. 4g+3: its right inputs are x-group g+1 (columns 4g+4..4g+7)
 // and its left inputs the upper half of x-group g-2 and the lower half of
 // x-group g-1. Chunk k computes output groups 8k..8k+7 (columns 32k..32k+31)
 // from x-groups 8k+1..8k+8, i.e. columns 32k+4..32k+35, staged in LDS; the
 // ring keeps the quantities of the last four x-groups (ring[g & 3] = group g).
-// The buffer range check applies to the per-lane offset only, so rows past the
-// image carry an out-of-range per-lane offset and the column is the scalar one.
+//
+// Lanes: each 16-lane DPP row holds five rows of the band as lane triples
+// (lane 15 idle); lane 3t+k runs cosine term k of its row, so a lane issues one
+// recurrence (4 instructions) per column, and the triple's first lane forms
+// (o1 + o3) + o5 with two DPP adds (lanes 3t+1 and 3t+2 shifted down; IEEE
+// addition commutes, so o3 + o1 is o1 + o3).
+//
+// The buffer range check applies to the per-lane offset only, so rows past
+// the image carry an out-of-range per-lane offset and the column is the
+// scalar one.
+typedef __attribute__((address_space(3))) vf4 lds_vf4;
+__device__ __forceinline__ vf4 to_vf4(float4 v) { return vf4{v.x, v.y, v.z, v.w}; }
+
 template <int Q>
 __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buffer_rsrc_t rb,
                                          __amdgpu_buffer_rsrc_t ro, int w, int pitch, int nrows,
-                                         float4* tile_a, float4* tile_b) {
+                                         lds_vf4* tile_a, lds_vf4* tile_b) {
   constexpr bool NA = Q != 1 && Q != 4;
   constexpr bool NB = Q == 1 || Q == 2 || Q == 4;
-  constexpr int T4 = kTileStride / 4;  // tile row stride in float4
+  constexpr int T4 = kTileStride / 4;        // tile row stride in float4
+  constexpr int NE = kRowsPerWave * 8;       // float4 of one chunk: rows x 8 segments
+  constexpr int NI = (NE + 63) / 64;         // coalesced instructions per chunk and plane
   const int lane = threadIdx.x;
-  const int lr = lane >> 3, ls = lane & 7;  // coalesced layout: row 8i+lr, segment ls
+  const int dl = lane & 15, tr = min(dl / 3, 4), k = dl - 3 * (dl / 3);
+  const int r = 5 * (lane >> 4) + tr;        // this lane's row in the band
+  const bool owner = dl < 15 && k == 0;      // the lane that stores its row's outputs
+  const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
+  const float mp = k == 0 ? kMulPrev1 : (k == 1 ? kMulPrev3 : kMulPrev5);
   const u32x4 z = {0u, 0u, 0u, 0u};
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  unsigned voff[8];
+  // coalesced layout: element e = 64 i + lane is row e / 8, segment e % 8;
+  // lanes past the chunk use the padding slot of tile row 0 and move nothing
+  unsigned voff[NI];
+  int eidx[NI];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int row = 8 * i + lr;
-    voff[i] = row < nrows ? (unsigned)(row * pitch * 4 + ls * 16) : kOffNone;
+  for (int i = 0; i < NI; ++i) {
+    const int e = 64 * i + lane, er = e >> 3, es = e & 7;
+    eidx[i] = e < NE ? er * T4 + es : T4 - 1;
+    voff[i] = e < NE && er < nrows ? (unsigned)(er * pitch * 4 + es * 16) : kOffNone;
   }
   const int glast = (w - 1) / 4;  // last output group
   const int nchunks = glast / 8 + 1;
-  u32x4 qa[8], qb[8];
-  auto load_chunk = [&](int k) {
+  u32x4 qa[NI], qb[NI];
+  auto load_chunk = [&](int kc) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      qa[i] = NA ? __builtin_amdgcn_raw_buffer_load_b128(ra, voff[i], k * 128 + 16, 0) : z;
-      qb[i] = NB ? __builtin_amdgcn_raw_buffer_load_b128(rb, voff[i], k * 128 + 16, 0) : z;
+    for (int i = 0; i < NI; ++i) {
+      qa[i] = NA ? __builtin_amdgcn_raw_buffer_load_b128(ra, voff[i], kc * 128 + 16, 0) : z;
+      qb[i] = NB ? __builtin_amdgcn_raw_buffer_load_b128(rb, voff[i], kc * 128 + 16, 0) : z;
     }
   };
   load_chunk(0);
@@ -331,70 +329,91 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
   {
     // x-group 0, then the steps n = -4 .. -1 (no outputs): right inputs
     // x-group 0, left inputs the zeros of x-groups -3 and -2.
-    const unsigned vrow = lane < nrows ? (unsigned)(lane * pitch * 4) : kOffNone;
+    const unsigned vrow = r < nrows ? (unsigned)(r * pitch * 4) : kOffNone;
     const float4 a = NA ? u2f(__builtin_amdgcn_raw_buffer_load_b128(ra, vrow, 0, 0)) : z4;
     const float4 b = NB ? u2f(__builtin_amdgcn_raw_buffer_load_b128(rb, vrow, 0, 0)) : z4;
     ring[0] = row_quantity<Q>(a, b);
     ring[1] = ring[2] = ring[3] = z4;
   }
-  RowState st;
-  st.step(0.f, ring[0].x);
-  st.step(0.f, ring[0].y);
-  st.step(0.f, ring[0].z);
-  st.step(0.f, ring[0].w);
-  for (int k = 0; k < nchunks; ++k) {
-    // stage x columns 32k+4 .. 32k+35 of rows 8i+lr into the padded tiles
+  float p = 0.f, q = 0.f;
+  // one step of this lane's term (cpu.rs:992-1010 for term k)
+  auto step = [&](float left, float right) {
+    const float sum = left + right;
+    float o = sum * mi;
+    o = fmaf(-1.0f, q, o);
+    q = p;
+    o = fmaf(mp, p, o);
+    p = o;
+    return o;
+  };
+  // (o1 + o3) + o5 on the triple's first lane: lanes 3t+1 and 3t+2 shifted
+  // down within the 16-lane row (row_shl:1 / :2, zero past the row's end,
+  // which only lanes that own no row see)
+  auto row_sum = [](float o) {
+    const float s1 = __builtin_amdgcn_update_dpp(0.0f, o, 0x101, 0xF, 0xF, true) + o;
+    return __builtin_amdgcn_update_dpp(0.0f, o, 0x102, 0xF, 0xF, true) + s1;
+  };
+  step(0.f, ring[0].x);
+  step(0.f, ring[0].y);
+  step(0.f, ring[0].z);
+  step(0.f, ring[0].w);
+  for (int kc = 0; kc < nchunks; ++kc) {
+    // stage x columns 32k+4 .. 32k+35 of the band's rows into the padded tiles
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      if (NA) tile_a[(8 * i + lr) * T4 + ls] = u2f(qa[i]);
-      if (NB) tile_b[(8 * i + lr) * T4 + ls] = u2f(qb[i]);
+    for (int i = 0; i < NI; ++i) {
+      if (NA) tile_a[eidx[i]] = to_vf4(u2f(qa[i]));
+      if (NB) tile_b[eidx[i]] = to_vf4(u2f(qb[i]));
     }
     wave_lds_order();
-    load_chunk(k + 1);  // past the last chunk: 0 or columns no output uses
-    // all tile reads in flight at once (LDS returns them in order), so the
-    // scan waits for each only as it reaches it
+    load_chunk(kc + 1);  // past the last chunk: 0 or columns no output uses
     float4 xa[8], xb[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      xa[j] = NA ? tile_a[lane * T4 + j] : z4;
-      xb[j] = NB ? tile_b[lane * T4 + j] : z4;
+      xa[j] = z4;
+      xb[j] = z4;
+      if (NA) xa[j] = to_f4(tile_a[r * T4 + j]);
+      if (NB) xb[j] = to_f4(tile_b[r * T4 + j]);
     }
     __builtin_amdgcn_sched_barrier(0);
     float4 out[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       ring[(j + 1) & 3] = row_quantity<Q>(xa[j], xb[j]);  // x-group 8k+j+1
-      const float4 r = ring[(j + 1) & 3], m2 = ring[(j + 2) & 3], m1 = ring[(j + 3) & 3];
-      out[j].x = st.step(m2.z, r.x);
-      out[j].y = st.step(m2.w, r.y);
-      out[j].z = st.step(m1.x, r.z);
-      out[j].w = st.step(m1.y, r.w);
+      const float4 rr = ring[(j + 1) & 3], m2 = ring[(j + 2) & 3], m1 = ring[(j + 3) & 3];
+      out[j].x = row_sum(step(m2.z, rr.x));
+      out[j].y = row_sum(step(m2.w, rr.y));
+      out[j].z = row_sum(step(m1.x, rr.z));
+      out[j].w = row_sum(step(m1.y, rr.w));
     }
-    // back through the tile: own row in, coalesced rows out; segments past
-    // the last output group are dropped with an out-of-range offset
+    // back through the tile: the owners' rows in, coalesced rows out; segments
+    // past the last output group are dropped with an out-of-range offset
     wave_lds_order();
+    if (owner) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) tile_a[lane * T4 + j] = out[j];
+      for (int j = 0; j < 8; ++j) tile_a[r * T4 + j] = to_vf4(out[j]);
+    }
     wave_lds_order();
-    const bool seg_ok = 8 * k + ls <= glast;
-    float4 v[8];
+    float4 v[NI];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = tile_a[(8 * i + lr) * T4 + ls];
+    for (int i = 0; i < NI; ++i) v[i] = to_f4(tile_a[eidx[i]]);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < NI; ++i) {
+      const bool seg_ok = 8 * kc + ((64 * i + lane) & 7) <= glast;
       const u32x4 u = {__float_as_uint(v[i].x), __float_as_uint(v[i].y), __float_as_uint(v[i].z),
                        __float_as_uint(v[i].w)};
-      __builtin_amdgcn_raw_buffer_store_b128(u, ro, seg_ok ? voff[i] : kOffNone, k * 128, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(u, ro, seg_ok ? voff[i] : kOffNone, kc * 128, 0);
     }
     wave_lds_order();
   }
 }
 
 template <int SIDES>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_rows(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
                                              const WorkItem* __restrict__ items) {
-  __shared__ float4 tiles[2][64 * kTileStride / 4];
+  __shared__ float4 tiles[2][kRowsPerWave * kTileStride / 4];
+  lds_vf4* tile_a = (lds_vf4*)tiles[0];
+  lds_vf4* tile_b = (lds_vf4*)tiles[1];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b / kNumQ, q = it.b % kNumQ;
@@ -408,11 +427,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   const __amdgpu_buffer_rsrc_t ro =
       buffer_rsrc(P.arena + S.hrow + (long long)(c * kNumQ + q) * S.plane + off, bytes);
   switch (q) {
-    case 0: row_scan<0>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
-    case 1: row_scan<1>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
-    case 2: row_scan<2>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
-    case 3: row_scan<3>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
-    default: row_scan<4>(ra, rb, ro, S.w, S.pitch, nrows, tiles[0], tiles[1]); break;
+    case 0: row_scan<0>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
+    case 1: row_scan<1>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
+    case 2: row_scan<2>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
+    case 3: row_scan<3>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
+    default: row_scan<4>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
   }
 }
 
