@@ -232,10 +232,8 @@ ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* 
   d.result = c->result;
   // The padding columns right of every plane row must read as zero (the row
   // scan takes them as the inputs past the image); no kernel writes them
-  // except the row pass's own outputs, which no later pass reads there. The
-  // result record's unit counter starts at zero.
+  // except the row pass's own outputs, which no later pass reads there.
   if ((e = hipMemset(c->arena, 0, arena_bytes)) != hipSuccess ||
-      (e = hipMemset(c->result, 0, kResultDoubles * sizeof(double))) != hipSuccess ||
       (e = hipMemcpy(c->d_desc, &d, sizeof(PairDesc), hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(c->d_items, p.items.data(), items_bytes, hipMemcpyHostToDevice)) != hipSuccess) {
     release(c);

@@ -334,13 +334,6 @@ ssim2_status ssim2_batch_create(const ssim2_batch_options* o, ssim2_batch** out)
     return bfail(SSIM2_OOM, std::string("batch allocation failed: ") + hipGetErrorString(e));
   }
   std::memset(b->h_index, 0, index_ints * sizeof(int));
-  // the result records' unit counters start at zero (the finalisation resets them)
-  if ((e = hipMemset(b->results, 0, size_t(o->max_pairs) * kResultDoubles * sizeof(double))) != hipSuccess) {
-    release(b);
-    delete b;
-    (void)hipGetLastError();
-    return bfail(SSIM2_HIP_ERROR, std::string("batch initialisation failed: ") + hipGetErrorString(e));
-  }
   *out = b;
   return SSIM2_OK;
 }

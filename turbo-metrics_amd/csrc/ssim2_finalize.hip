@@ -1,18 +1,17 @@
-// Finalisation shared by the exact path and the fast form, in one launch:
-// per-pair, per-(scale, channel) reduction of the partial-sum records, then
-// the 108-feature score (crates/ssimulacra2-cuda/examples/cpu.rs:633-634,
-// 676-679 for the averages, 840-868 for the weighted sum and transform), in
-// f64 on the device. One block per (pair, scale*3 + channel) unit,
-// persistent: 256 threads stride over the records, then a fixed-order tree
-// (wave shuffles, waves in sequence), so the order does not depend on which
-// other pairs share the launch. Each unit publishes its sums and counts
-// itself in the pair's result record; the block that counts a pair's last
-// unit evaluates the 108 features in parallel, forms the weighted sum in
-// cpu.rs's order and resets the counter to zero for the next launch.
+// Finalisation shared by the exact path and the fast form: per-pair,
+// per-(scale, channel) reduction of the partial-sum records, then the
+// 108-feature score (crates/ssimulacra2-cuda/examples/cpu.rs:633-634, 676-679
+// for the averages, 840-868 for the weighted sum and transform), in f64 on the
+// device. Two launches:
+//   k_reduce_groups  one block per (pair, scale*3 + channel) unit, persistent:
+//                    256 threads stride over the records, then a fixed-order
+//                    tree (wave shuffles, waves in sequence); the order does
+//                    not depend on which other pairs share the launch.
+//   k_score          one wave per pair: the 108 features in parallel, then the
+//                    weighted sum in cpu.rs's order on one lane.
 #include <algorithm>
 
 #include "ssim2_internal.h"
-#include "ssim2_plan.h"
 #include "ssim2_tables.h"
 
 namespace ssim2 {
@@ -36,11 +35,9 @@ __device__ __forceinline__ void lds_sync() {
 
 }  // namespace
 
-__global__ __launch_bounds__(kThreads) void k_finalize(const PairDesc* __restrict__ pairs,
-                                                       const int* __restrict__ meta, int npairs_static) {
+__global__ __launch_bounds__(kThreads) void k_reduce_groups(const PairDesc* __restrict__ pairs,
+                                                            const int* __restrict__ meta, int npairs_static) {
   __shared__ double red[kThreads / 64][kNumSums];
-  __shared__ double feat[kSumsPerPair];
-  __shared__ int last;
   const int npairs = meta ? meta[0] : npairs_static;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int u = blockIdx.x; u < npairs * kGroups; u += gridDim.x) {
@@ -54,7 +51,7 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const PairDesc* __restric
 #pragma unroll
         for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
     }
-    lds_sync();  // previous unit's red[] / feat[] reads are done
+    lds_sync();  // previous unit's red[] reads are done
 #pragma unroll
     for (int q = 0; q < kNumSums; ++q) {
       const double r = wave_sum_f(v[q]);
@@ -70,24 +67,28 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const PairDesc* __restric
       const int ss = t / 2;
       P.result[1 + kSumsPerPair + t] = ss < P.nscales ? (double)(t & 1 ? P.sc[ss].h : P.sc[ss].w) : 0.0;
     }
-    // publish, then count this unit; the pair's last unit goes on to score it
-    __threadfence();
-    lds_sync();
-    unsigned* counter = reinterpret_cast<unsigned*>(P.result + kResultCounter);
-    if (t == 0) last = atomicAdd(counter, 1u) == (unsigned)(kGroups - 1);
-    lds_sync();
-    if (!last) continue;
-    __threadfence();  // the other units' sums are visible from here
+  }
+}
+
+__global__ __launch_bounds__(64) void k_score(const PairDesc* __restrict__ pairs, const int* __restrict__ meta,
+                                              int npairs_static) {
+  __shared__ double feat[kSumsPerPair];
+  const int npairs = meta ? meta[0] : npairs_static;
+  const int lane = threadIdx.x;
+  for (int pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
+    const PairDesc& P = pairs[pi];
     const int ns = P.nscales, nf = 3 * ns * 6;
     // feature k in cpu.rs order: channel, present scale, {L1 of the three maps, L4 of the three}
-    if (t < nf) {
-      const int fc = t / (ns * 6), r = t % (ns * 6), fs = r / 6, j = r % 6;
-      const double opp = 1.0 / (double)((long long)P.sc[fs].w * P.sc[fs].h);
-      const double* q = P.result + 1 + (fs * 3 + fc) * kNumSums;
-      feat[t] = j < 3 ? fabs(opp * q[2 * j]) : fabs(sqrt(sqrt(opp * q[2 * (j - 3) + 1])));
+    for (int k = lane; k < nf; k += 64) {
+      const int c = k / (ns * 6), r = k % (ns * 6), s = r / 6, j = r % 6;
+      const double opp = 1.0 / (double)((long long)P.sc[s].w * P.sc[s].h);
+      const double* q = P.result + 1 + (s * 3 + c) * kNumSums;
+      feat[k] = j < 3 ? fabs(opp * q[2 * j]) : fabs(sqrt(sqrt(opp * q[2 * (j - 3) + 1])));
     }
-    lds_sync();
-    if (t == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (lane == 0) {
       double ssim = 0.0;
       for (int k = 0; k < nf; ++k) ssim = fma(c_weight_fin[k], feat[k], ssim);
       ssim *= 0.9562382616834844;
@@ -95,15 +96,20 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const PairDesc* __restric
                  fma(2.326765642916932, ssim, -0.020884521182843837 * ssim * ssim));
       ssim = ssim > 0.0 ? fma(pow(ssim, 0.6276336467831387), -10.0, 100.0) : 100.0;
       P.result[0] = ssim;
-      atomicExch(counter, 0u);
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   }
 }
 
 hipError_t launch_finalize_pairs(const PairDesc* pairs, const int* meta, int npairs_static, int grid_units,
                                  int max_pairs, hipStream_t stream) {
-  (void)max_pairs;
-  hipLaunchKernelGGL(k_finalize, dim3(grid_units), dim3(kThreads), 0, stream, pairs, meta, npairs_static);
+  hipLaunchKernelGGL(k_reduce_groups, dim3(grid_units), dim3(kThreads), 0, stream, pairs, meta, npairs_static);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_score, dim3(std::max(1, std::min(max_pairs, 1024))), dim3(64), 0, stream, pairs, meta,
+                     npairs_static);
   return hipGetLastError();
 }
 

@@ -10,10 +10,8 @@ namespace ssim2 {
 
 constexpr int kMaxDim = 32768;
 constexpr long long kMaxPixels = 1LL << 28;
-// Device result record per pair: score, 108 sums, 12 dims, and one slot used
-// as the finalisation's unit counter (zero between launches).
-constexpr int kResultCounter = 1 + kSumsPerPair + 2 * kMaxScales;
-constexpr int kResultDoubles = kResultCounter + 1;
+// Device result record per pair: score, 108 sums, 12 dims.
+constexpr int kResultDoubles = 1 + kSumsPerPair + 2 * kMaxScales;
 // Planes per scale in the arena: 6 XYB + 15 row-pass + 9 cached-reference.
 constexpr int kPlanesPerScale = 6 + 3 * kNumQ + 9;
 
