@@ -5,14 +5,16 @@
 //    in exact arithmetic (crates/ssimulacra2-cuda-kernel/build.rs:28-140; the
 //    identity is checked in tests/test_oracle.py), evaluated as a separable
 //    tile convolution in LDS instead of a sequential scan;
-//  * the five blurred quantities are formed in shifted coordinates
+//  * the blurred quantities are formed in shifted coordinates
 //    x' = x - shift(channel); the statistics are rebuilt with the exact
 //    corrections for the zero-padded border (b1 = blur of ones, e = 1 - b1):
 //        var = blur(x'^2) - blur(x')^2 + e*c*(2*blur(x') + c*b1)
 //        cov = blur(r'd') - blur(r')*blur(d') + e*c*(blur(r') + blur(d') + c*b1)
 //        x - mu = (x' - blur(x')) + c*e,  mu1 - mu2 = blur(r') - blur(d')
-//    which removes most f32 cancellation in sigma - mu^2;
-//  * the cube root is exp2(log2(x)/3) plus one Newton step (f32), the error
+//    which removes most f32 cancellation in sigma - mu^2; the maps use the
+//    two variances only as var1 + var2, so r'^2 + d'^2 is blurred as one
+//    quantity (four blurs per channel: r'^2 + d'^2, r'd', r', d');
+//  * the cube root is exp2(log2(x)/3) on the hardware log2/exp2, the error
 //    maps are f32 with hardware reciprocals for their two quotients, tile sums
 //    are f32 within a wave and f64 across waves and tiles;
 //  * XYB of scales >= 1 is stored in HBM as (value - shift) in the storage type
@@ -27,6 +29,8 @@
 //                   XYB tile + 4 px halo -> products -> horizontal FIR ->
 //                   vertical FIR -> maps -> f64 block sums.
 //   finalisation    k_reduce_groups + k_score (ssim2_finalize.hip).
+#include <type_traits>
+
 #include "ssim2_input.h"
 #include "ssim2_internal.h"
 #include "ssim2_math.h"
@@ -46,11 +50,12 @@ constexpr float kH0 = 0x1.0ef8d6p-2f, kH1 = 0x1.b413e8p-3f, kH2 = 0x1.bfd672p-4f
 // Y 0.010..0.855, B 0.272..0.938), exactly representable.
 __device__ __forceinline__ float xyb_shift(int c) { return c == 0 ? 0.5f : (c == 1 ? 0.4375f : 0.625f); }
 
+// Cube root as exp2(log2(x) / 3) on the hardware log2/exp2 (about 1 ulp
+// each): a few ulp in all, far below the binary16 storage quantum and the
+// FIR's own rounding, and two quarter-rate operations instead of three.
 __device__ __forceinline__ float cbrt_fast(float x) {
-  if (!(x > 0.0f)) return 0.0f;
-  float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * 0.333333343f);
-  const float r = __builtin_amdgcn_rcpf(y * y);
-  return fmaf(fmaf(x, r, -y), 0.333333343f, y);
+  const float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * 0.333333343f);
+  return x > 0.0f ? y : 0.0f;
 }
 
 __device__ __forceinline__ void xyb_fast(float r, float g, float b, float& X, float& Y, float& B) {
@@ -255,8 +260,11 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
                                                         const int* __restrict__ meta) {
   static_assert(TX % 4 == 0 && TY % RU == 0, "tile shape");
   constexpr int IW = TX + 8, IH = TY + 8;
+  // Blurred quantities: r'^2 + d'^2, r'd', r', d'. The maps use the two
+  // variances only through their sum, so one blur serves both.
+  constexpr int kQ = 4;
   __shared__ float xs[2][3][IH][IW + 1];  // shifted XYB, zero outside the image
-  __shared__ float hq[5][IH][TX + 1];     // horizontal pass, one channel at a time
+  __shared__ float hq[kQ][IH][TX + 1];    // horizontal pass, one channel at a time
   __shared__ double red[kThreads / 64][18];
   const int npairs = meta[0], n = prefix[npairs];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -271,6 +279,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
     const int w = S.w, h = S.h;
     const int tx = tile % S.tiles_x, ty = tile / S.tiles_x;
     const int x0 = tx * TX, y0 = ty * TY;
+    const bool interior = x0 >= 4 && x0 + TX <= w - 4 && y0 >= 4 && y0 + TY <= h - 4;
     fast_barrier();  // previous item's LDS reads are done
 
     // Phase 1: shifted XYB of the tile and its 4 px halo.
@@ -312,7 +321,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
 
     for (int c = 0; c < 3; ++c) {
       fast_barrier();  // xs complete (c = 0) / previous channel's hq reads done
-      // Phase 2: horizontal FIR of the five quantities, 4 outputs per unit.
+      // Phase 2: horizontal FIR of the four quantities, 4 outputs per unit.
       for (int e = tid; e < IH * (TX / 4); e += kThreads) {
         const int iy = e / (TX / 4), ox0 = (e % (TX / 4)) * 4;
         float a[12], d[12];
@@ -322,11 +331,11 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
           d[k] = xs[1][c][iy][ox0 + k];
         }
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < kQ; ++q) {
           float qv[12];
 #pragma unroll
           for (int k = 0; k < 12; ++k)
-            qv[k] = q == 0 ? a[k] * a[k] : (q == 1 ? d[k] * d[k] : (q == 2 ? a[k] * d[k] : (q == 3 ? a[k] : d[k])));
+            qv[k] = q == 0 ? a[k] * a[k] + d[k] * d[k] : (q == 1 ? a[k] * d[k] : (q == 2 ? a[k] : d[k]));
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             float r = kH4 * (qv[j] + qv[j + 8]);
@@ -340,59 +349,84 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
       }
       fast_barrier();
       // Phase 3: vertical FIR (RU rows per unit), statistics, error maps.
+      // A tile whose outputs all lie 4 px or more inside the image has
+      // b1 = 1 and e = 0 everywhere, so its statistics need none of the
+      // zero-padding terms (the choice is uniform over the block).
       const float sh = xyb_shift(c);
-      for (int e = tid; e < (TY / RU) * TX; e += kThreads) {
-        const int oy0 = (e / TX) * RU, ox = e % TX;
-        float bq[5][RU];
+      auto maps = [&](auto edge_tag) {
+        constexpr bool kEdge = decltype(edge_tag)::value;
+        for (int e = tid; e < (TY / RU) * TX; e += kThreads) {
+          const int oy0 = (e / TX) * RU, ox = e % TX;
+          float bq[kQ][RU];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          float col[RU + 8];
+          for (int q = 0; q < kQ; ++q) {
+            float col[RU + 8];
 #pragma unroll
-          for (int k = 0; k < RU + 8; ++k) col[k] = hq[q][oy0 + k][ox];
+            for (int k = 0; k < RU + 8; ++k) col[k] = hq[q][oy0 + k][ox];
+#pragma unroll
+            for (int j = 0; j < RU; ++j) {
+              float r = kH4 * (col[j] + col[j + 8]);
+              r = fmaf(kH3, col[j + 1] + col[j + 7], r);
+              r = fmaf(kH2, col[j + 2] + col[j + 6], r);
+              r = fmaf(kH1, col[j + 3] + col[j + 5], r);
+              r = fmaf(kH0, col[j + 4], r);
+              bq[q][j] = r;
+            }
+          }
+          const int gx = x0 + ox;
+          const float b1x = kEdge ? tap_sum(gx, w) : 1.0f;
 #pragma unroll
           for (int j = 0; j < RU; ++j) {
-            float r = kH4 * (col[j] + col[j + 8]);
-            r = fmaf(kH3, col[j + 1] + col[j + 7], r);
-            r = fmaf(kH2, col[j + 2] + col[j + 6], r);
-            r = fmaf(kH1, col[j + 3] + col[j + 5], r);
-            r = fmaf(kH0, col[j + 4], r);
-            bq[q][j] = r;
+            const float ssum = bq[0][j], s12 = bq[1][j], m1 = bq[2][j], m2 = bq[3][j];
+            // var1 + var2 and cov (plus, near the border, the zero-padding
+            // terms e*c*(2*(m1 + m2) + 2*c*b1) and e*c*((m1 + m2) + c*b1)).
+            // Products rounded on their own: for identical images r'^2 + d'^2
+            // is exactly 2*r'd' (and so are its blur and m1^2 + m2^2), which
+            // makes var1 + var2 exactly 2*cov and the quotient exactly 1.
+            float vsum = ssum - (m1 * m1 + m2 * m2), cov = s12 - m1 * m2;
+            float ec = 0.0f;
+            if constexpr (kEdge) {
+              const int gy = y0 + oy0 + j;
+              if (gx >= w || gy >= h) continue;
+              const float b1 = b1x * tap_sum(gy, h);
+              ec = (1.0f - b1) * sh;
+              const float t = (m1 + m2) + sh * b1;
+              vsum = fmaf(2.0f * ec, t, vsum);
+              cov = fmaf(ec, t, cov);
+            }
+            const float md = m1 - m2;
+            const float num_m = fmaf(md, -md, 1.0f);
+            const float num_s = fmaf(2.0f, cov, kC2);
+            const float den = vsum + kC2;
+            // quotients by hardware reciprocal; equal operands (identical
+            // images) keep their exact quotient 1, so identical pairs still
+            // score 100
+            const float nm = num_m * num_s;
+            const float dd = fmaxf(1.0f - (nm == den ? 1.0f : nm * __builtin_amdgcn_rcpf(den)), 0.0f);
+            const float d2 = dd * dd;
+            acc[c][0] += dd;
+            acc[c][1] += d2 * d2;
+            const float src = xs[0][c][oy0 + j + 4][ox + 4], dis = xs[1][c][oy0 + j + 4][ox + 4];
+            float rn = dis - m2, rd = src - m1;
+            if constexpr (kEdge) {
+              rn += ec;
+              rd += ec;
+            }
+            const float en = 1.0f + fabsf(rn), ed = 1.0f + fabsf(rd);
+            const float d1 = en == ed ? 0.0f : fmaf(en, __builtin_amdgcn_rcpf(ed), -1.0f);
+            const float art = fmaxf(d1, 0.0f), det = fmaxf(-d1, 0.0f);
+            const float a2 = art * art, e2 = det * det;
+            acc[c][2] += art;
+            acc[c][3] += a2 * a2;
+            acc[c][4] += det;
+            acc[c][5] += e2 * e2;
           }
         }
-        const int gx = x0 + ox;
-        const float b1x = tap_sum(gx, w);
-#pragma unroll
-        for (int j = 0; j < RU; ++j) {
-          const int gy = y0 + oy0 + j;
-          if (gx >= w || gy >= h) continue;
-          const float b1 = b1x * tap_sum(gy, h);
-          const float ec = (1.0f - b1) * sh, cb = sh * b1;
-          const float s11 = bq[0][j], s22 = bq[1][j], s12 = bq[2][j], m1 = bq[3][j], m2 = bq[4][j];
-          const float var1 = (s11 - m1 * m1) + ec * (2.0f * m1 + cb);
-          const float var2 = (s22 - m2 * m2) + ec * (2.0f * m2 + cb);
-          const float cov = (s12 - m1 * m2) + ec * ((m1 + m2) + cb);
-          const float md = m1 - m2;
-          const float num_m = fmaf(md, -md, 1.0f);
-          const float num_s = fmaf(2.0f, cov, kC2);
-          const float den = (var1 + var2) + kC2;
-          // quotients by hardware reciprocal; equal operands (identical images)
-          // keep their exact quotient 1, so identical pairs still score 100
-          const float nm = num_m * num_s;
-          const float dd = fmaxf(1.0f - (nm == den ? 1.0f : nm * __builtin_amdgcn_rcpf(den)), 0.0f);
-          const float d2 = dd * dd;
-          acc[c][0] += dd;
-          acc[c][1] += d2 * d2;
-          const float src = xs[0][c][oy0 + j + 4][ox + 4], dis = xs[1][c][oy0 + j + 4][ox + 4];
-          const float en = 1.0f + fabsf((dis - m2) + ec), ed = 1.0f + fabsf((src - m1) + ec);
-          const float d1 = en == ed ? 0.0f : en * __builtin_amdgcn_rcpf(ed) - 1.0f;
-          const float art = fmaxf(d1, 0.0f), det = fmaxf(-d1, 0.0f);
-          const float a2 = art * art, e2 = det * det;
-          acc[c][2] += art;
-          acc[c][3] += a2 * a2;
-          acc[c][4] += det;
-          acc[c][5] += e2 * e2;
-        }
-      }
+      };
+      if (interior)
+        maps(std::false_type{});
+      else
+        maps(std::true_type{});
     }
     // Block sums in a fixed order: f32 within each wave (DPP), then f64 over
     // the waves in sequence.
