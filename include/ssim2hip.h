@@ -219,7 +219,10 @@ typedef struct ssim2_batch ssim2_batch;
 ssim2_status ssim2_batch_create(const ssim2_batch_options* options, ssim2_batch** out);
 ssim2_status ssim2_batch_destroy(ssim2_batch* batch);
 ssim2_status ssim2_batch_mem_usage(const ssim2_batch* batch, size_t* bytes);
-/* Enqueue one batch on `stream` (waits first for this object's previous batch). */
+/* Enqueue one batch on `stream`. The host stages it while the previous batch
+ * runs (two staging slots: the call blocks only until the batch before the
+ * previous one has finished); on the device it starts after the previous
+ * batch, on any stream. Scores and sums refer to the last batch enqueued. */
 ssim2_status ssim2_batch_compute_async(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
                                        ssim2_stream stream);
 /* As ssim2_batch_compute_async for YUV 4:2:0 frame pairs (fused ingest). */

@@ -91,18 +91,24 @@ struct ssim2_batch {
   size_t arena_cap = 0, part_cap = 0;
   unsigned char* arena = nullptr;
   double* partials = nullptr;
-  double* results = nullptr;
-  PairDesc* d_desc = nullptr;
-  int* d_index = nullptr;  // [meta(4) | pyr prefix (max+1) | tile prefix (max+1)]
-  PairDesc* h_desc = nullptr;
-  int* h_index = nullptr;
-  double* h_results = nullptr;
-  hipEvent_t done = nullptr;
+  // Two staging slots: the host stages batch k+1 while batch k runs; batch
+  // k-1's slot is the one reused, so staging waits on it at most.
+  struct Slot {
+    double* results = nullptr;
+    PairDesc* d_desc = nullptr;
+    int* d_index = nullptr;  // [meta(4) | pyr prefix (max+1) | tile prefix (max+1)]
+    PairDesc* h_desc = nullptr;
+    int* h_index = nullptr;
+    double* h_results = nullptr;
+    hipEvent_t done = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_cfg = -1, graph_mult = -1;
+    bool pending = false;
+    int n = 0;
+  } slot[2];
+  int cur = 0;     // slot of the last submitted batch
+  int staged = 1;  // slot filled by the last stage()
   hipStream_t cap_stream = nullptr;
-  hipGraphExec_t graph = nullptr;
-  int graph_cfg = -1, graph_mult = -1;
-  bool pending = false;
-  int last_n = 0;
   int cus = 256;
   size_t bytes = 0;
   std::string tune_report;
@@ -115,53 +121,55 @@ int* pyr_prefix(ssim2_batch* b, int* base) { return base + 4; }
 int* tile_prefix(ssim2_batch* b, int* base) { return base + 4 + (b->max_pairs + 1); }
 
 void release(ssim2_batch* b) {
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (ssim2_batch::Slot& S : b->slot) {
+    if (S.graph) (void)hipGraphExecDestroy(S.graph);
+    if (S.done) (void)hipEventDestroy(S.done);
+    if (S.results) (void)hipFree(S.results);
+    if (S.d_desc) (void)hipFree(S.d_desc);
+    if (S.d_index) (void)hipFree(S.d_index);
+    if (S.h_desc) (void)hipHostFree(S.h_desc);
+    if (S.h_index) (void)hipHostFree(S.h_index);
+    if (S.h_results) (void)hipHostFree(S.h_results);
+  }
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-  if (b->done) (void)hipEventDestroy(b->done);
   if (b->arena) (void)hipFree(b->arena);
   if (b->partials) (void)hipFree(b->partials);
-  if (b->results) (void)hipFree(b->results);
-  if (b->d_desc) (void)hipFree(b->d_desc);
-  if (b->d_index) (void)hipFree(b->d_index);
-  if (b->h_desc) (void)hipHostFree(b->h_desc);
-  if (b->h_index) (void)hipHostFree(b->h_index);
-  if (b->h_results) (void)hipHostFree(b->h_results);
 }
 
-ssim2_status enqueue_kernels(ssim2_batch* b, hipStream_t s, int cfg, int mult) {
-  const int* meta = meta_ptr(b->d_index);
-  BHIP(launch_fast_pyramid(b->storage, b->d_desc, pyr_prefix(b, b->d_index), meta, b->cus * 4, s));
-  BHIP(launch_fast_tile(b->storage, cfg, b->d_desc, tile_prefix(b, b->d_index), meta, b->cus * mult, s));
-  BHIP(launch_fast_finalize(b->d_desc, meta, b->cus * 4, (int)b->max_pairs, s));
+ssim2_status enqueue_kernels(ssim2_batch* b, const ssim2_batch::Slot& S, hipStream_t s, int cfg, int mult) {
+  const int* meta = meta_ptr(S.d_index);
+  BHIP(launch_fast_pyramid(b->storage, S.d_desc, pyr_prefix(b, S.d_index), meta, b->cus * 4, s));
+  BHIP(launch_fast_tile(b->storage, cfg, S.d_desc, tile_prefix(b, S.d_index), meta, b->cus * mult, s));
+  BHIP(launch_fast_finalize(S.d_desc, meta, b->cus * 4, (int)b->max_pairs, s));
   return SSIM2_OK;
 }
 
 // Capture the three persistent launches once per (tile shape, grid size); the
 // grids are fixed and all sizes come from device memory, so the same graph
 // runs every batch.
-ssim2_status ensure_graph(ssim2_batch* b) {
-  if (b->graph && b->graph_cfg == b->tile_cfg && b->graph_mult == b->grid_mult) return SSIM2_OK;
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
+ssim2_status ensure_graph(ssim2_batch* b, ssim2_batch::Slot& S) {
+  if (S.graph && S.graph_cfg == b->tile_cfg && S.graph_mult == b->grid_mult) return SSIM2_OK;
+  if (S.graph) {
+    (void)hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
   }
   hipGraph_t g = nullptr;
   BHIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-  ssim2_status st = enqueue_kernels(b, b->cap_stream, b->tile_cfg, b->grid_mult);
+  ssim2_status st = enqueue_kernels(b, S, b->cap_stream, b->tile_cfg, b->grid_mult);
   hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
   if (st != SSIM2_OK) {
     if (g) (void)hipGraphDestroy(g);
     return st;
   }
   if (e != hipSuccess) return bfail(SSIM2_HIP_ERROR, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    b->graph = nullptr;
+    S.graph = nullptr;
     return bfail(SSIM2_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
-  b->graph_cfg = b->tile_cfg;
-  b->graph_mult = b->grid_mult;
+  S.graph_cfg = b->tile_cfg;
+  S.graph_mult = b->grid_mult;
   return SSIM2_OK;
 }
 
@@ -171,13 +179,18 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
   if (n > b->max_pairs)
     return bfail(SSIM2_BATCH_SIZE, std::to_string(n) + " pairs exceed the batch capacity of " +
                                        std::to_string(b->max_pairs));
-  if (b->pending) BHIP(hipEventSynchronize(b->done));  // staging and device buffers are free
-  b->pending = false;
+  // the slot not used by the last submitted batch; its own previous batch
+  // (two submissions ago) must have finished with its buffers
+  const int si = b->cur ^ 1;
+  ssim2_batch::Slot& SL = b->slot[si];
+  if (SL.pending) BHIP(hipEventSynchronize(SL.done));
+  SL.pending = false;
+  b->staged = si;
   size_t arena = 0;
   int parts = 0, pyr = 0, tiles = 0;
   uint64_t px = 0;
-  int* pp = pyr_prefix(b, b->h_index);
-  int* tp = tile_prefix(b, b->h_index);
+  int* pp = pyr_prefix(b, SL.h_index);
+  int* tp = tile_prefix(b, SL.h_index);
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t qw, qh;
     if (ypairs) {
@@ -232,8 +245,8 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
     }
     d.arena = reinterpret_cast<float*>(b->arena);
     d.partials = b->partials;
-    d.result = b->results + size_t(i) * kResultDoubles;
-    b->h_desc[i] = d;
+    d.result = SL.results + size_t(i) * kResultDoubles;
+    SL.h_desc[i] = d;
     pp[i] = pyr;
     tp[i] = tiles;
     pyr += fp.npyr;
@@ -246,7 +259,7 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
                                        std::to_string(b->max_pixels) + " px");
   pp[n] = pyr;
   tp[n] = tiles;
-  int* meta = meta_ptr(b->h_index);
+  int* meta = meta_ptr(SL.h_index);
   meta[0] = (int)n;
   meta[1] = pyr;
   meta[2] = tiles;
@@ -256,20 +269,26 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
 
 ssim2_status submit(ssim2_batch* b, uint32_t n, hipStream_t s, bool use_graph, int cfg, int mult) {
   const size_t index_ints = 4 + 2 * size_t(b->max_pairs + 1);
-  BHIP(hipMemcpyAsync(b->d_desc, b->h_desc, n * sizeof(PairDesc), hipMemcpyHostToDevice, s));
-  BHIP(hipMemcpyAsync(b->d_index, b->h_index, index_ints * sizeof(int), hipMemcpyHostToDevice, s));
+  ssim2_batch::Slot& S = b->slot[b->staged];
+  ssim2_batch::Slot& prev = b->slot[b->cur];
+  // The arena and partial sums are shared by the slots: a batch starts after
+  // the previous one (a no-op on the same stream, an ordering across streams).
+  if (b->cur != b->staged && prev.pending) BHIP(hipStreamWaitEvent(s, prev.done, 0));
+  BHIP(hipMemcpyAsync(S.d_desc, S.h_desc, n * sizeof(PairDesc), hipMemcpyHostToDevice, s));
+  BHIP(hipMemcpyAsync(S.d_index, S.h_index, index_ints * sizeof(int), hipMemcpyHostToDevice, s));
   if (use_graph) {
-    ssim2_status st = ensure_graph(b);
+    ssim2_status st = ensure_graph(b, S);
     if (st != SSIM2_OK) return st;
-    BHIP(hipGraphLaunch(b->graph, s));
+    BHIP(hipGraphLaunch(S.graph, s));
   } else {
-    ssim2_status st = enqueue_kernels(b, s, cfg, mult);
+    ssim2_status st = enqueue_kernels(b, S, s, cfg, mult);
     if (st != SSIM2_OK) return st;
   }
-  BHIP(hipMemcpyAsync(b->h_results, b->results, n * kResultDoubles * sizeof(double), hipMemcpyDeviceToHost, s));
-  BHIP(hipEventRecord(b->done, s));
-  b->pending = true;
-  b->last_n = (int)n;
+  BHIP(hipMemcpyAsync(S.h_results, S.results, n * kResultDoubles * sizeof(double), hipMemcpyDeviceToHost, s));
+  BHIP(hipEventRecord(S.done, s));
+  S.pending = true;
+  S.n = (int)n;
+  b->cur = b->staged;
   return SSIM2_OK;
 }
 
@@ -299,7 +318,8 @@ ssim2_status ssim2_batch_create(const ssim2_batch_options* o, ssim2_batch** out)
   b->part_cap = size_t(o->max_pixels) * 3 / 128 + size_t(o->max_pairs) * 3 * 2048;
   const size_t index_ints = 4 + 2 * size_t(o->max_pairs + 1);
   b->bytes = b->arena_cap + b->part_cap * kNumSums * sizeof(double) +
-             size_t(o->max_pairs) * (kResultDoubles * sizeof(double) + sizeof(PairDesc)) + index_ints * sizeof(int);
+             2 * (size_t(o->max_pairs) * (kResultDoubles * sizeof(double) + sizeof(PairDesc)) +
+                  index_ints * sizeof(int));
   size_t budget = (size_t)o->memory_budget;
   if (budget == 0) {
     size_t fb = 0, tb = 0;
@@ -317,30 +337,36 @@ ssim2_status ssim2_batch_create(const ssim2_batch_options* o, ssim2_batch** out)
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) b->cus = v;
   }
   hipError_t e;
-  if ((e = hipMalloc(&b->arena, b->arena_cap)) != hipSuccess ||
-      (e = hipMalloc(&b->partials, b->part_cap * kNumSums * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&b->results, size_t(o->max_pairs) * kResultDoubles * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&b->d_desc, size_t(o->max_pairs) * sizeof(PairDesc))) != hipSuccess ||
-      (e = hipMalloc(&b->d_index, index_ints * sizeof(int))) != hipSuccess ||
-      (e = hipHostMalloc(&b->h_desc, size_t(o->max_pairs) * sizeof(PairDesc), hipHostMallocDefault)) != hipSuccess ||
-      (e = hipHostMalloc(&b->h_index, index_ints * sizeof(int), hipHostMallocDefault)) != hipSuccess ||
-      (e = hipHostMalloc(&b->h_results, size_t(o->max_pairs) * kResultDoubles * sizeof(double),
-                         hipHostMallocDefault)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess) {
+  if ((e = hipMalloc(&b->arena, b->arena_cap)) == hipSuccess &&
+      (e = hipMalloc(&b->partials, b->part_cap * kNumSums * sizeof(double))) == hipSuccess)
+    e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking);
+  for (ssim2_batch::Slot& S : b->slot) {
+    if (e != hipSuccess) break;
+    if ((e = hipMalloc(&S.results, size_t(o->max_pairs) * kResultDoubles * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc(&S.d_desc, size_t(o->max_pairs) * sizeof(PairDesc))) != hipSuccess ||
+        (e = hipMalloc(&S.d_index, index_ints * sizeof(int))) != hipSuccess ||
+        (e = hipHostMalloc(&S.h_desc, size_t(o->max_pairs) * sizeof(PairDesc), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&S.h_index, index_ints * sizeof(int), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&S.h_results, size_t(o->max_pairs) * kResultDoubles * sizeof(double),
+                           hipHostMallocDefault)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&S.done, hipEventDisableTiming)) != hipSuccess)
+      break;
+    std::memset(S.h_index, 0, index_ints * sizeof(int));
+  }
+  if (e != hipSuccess) {
     release(b);
     delete b;
     (void)hipGetLastError();
     return bfail(SSIM2_OOM, std::string("batch allocation failed: ") + hipGetErrorString(e));
   }
-  std::memset(b->h_index, 0, index_ints * sizeof(int));
   *out = b;
   return SSIM2_OK;
 }
 
 ssim2_status ssim2_batch_destroy(ssim2_batch* b) {
   if (!b) return SSIM2_OK;
-  if (b->pending) (void)hipEventSynchronize(b->done);
+  for (ssim2_batch::Slot& S : b->slot)
+    if (S.pending) (void)hipEventSynchronize(S.done);
   release(b);
   delete b;
   return SSIM2_OK;
@@ -370,21 +396,23 @@ ssim2_status ssim2_batch_compute_yuv_async(ssim2_batch* b, const ssim2_yuv_pair*
 
 ssim2_status ssim2_batch_get_scores(ssim2_batch* b, double* scores, uint32_t capacity) {
   if (!b || !scores) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
-  if (!b->pending && b->last_n == 0) return bfail(SSIM2_INVALID_ARGUMENT, "no batch has been submitted");
-  if (capacity < (uint32_t)b->last_n) return bfail(SSIM2_BATCH_SIZE, "score buffer too small");
-  if (b->pending) BHIP(hipEventSynchronize(b->done));
-  b->pending = false;
-  for (int i = 0; i < b->last_n; ++i) scores[i] = b->h_results[size_t(i) * kResultDoubles];
+  ssim2_batch::Slot& S = b->slot[b->cur];
+  if (!S.pending && S.n == 0) return bfail(SSIM2_INVALID_ARGUMENT, "no batch has been submitted");
+  if (capacity < (uint32_t)S.n) return bfail(SSIM2_BATCH_SIZE, "score buffer too small");
+  if (S.pending) BHIP(hipEventSynchronize(S.done));
+  S.pending = false;
+  for (int i = 0; i < S.n; ++i) scores[i] = S.h_results[size_t(i) * kResultDoubles];
   return SSIM2_OK;
 }
 
 ssim2_status ssim2_batch_query(ssim2_batch* b, int* done) {
   if (!b || !done) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
-  if (!b->pending) {
+  const ssim2_batch::Slot& S = b->slot[b->cur];
+  if (!S.pending) {
     *done = 1;
     return SSIM2_OK;
   }
-  hipError_t e = hipEventQuery(b->done);
+  hipError_t e = hipEventQuery(S.done);
   if (e == hipErrorNotReady) {
     *done = 0;
     return SSIM2_OK;
@@ -396,10 +424,11 @@ ssim2_status ssim2_batch_query(ssim2_batch* b, int* done) {
 
 ssim2_status ssim2_batch_get_sums(ssim2_batch* b, uint32_t index, double* sums108, int* dims12) {
   if (!b) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
-  if (index >= (uint32_t)b->last_n) return bfail(SSIM2_INVALID_ARGUMENT, "pair index out of range");
-  if (b->pending) BHIP(hipEventSynchronize(b->done));
-  b->pending = false;
-  const double* r = b->h_results + size_t(index) * kResultDoubles;
+  ssim2_batch::Slot& S = b->slot[b->cur];
+  if (index >= (uint32_t)S.n) return bfail(SSIM2_INVALID_ARGUMENT, "pair index out of range");
+  if (S.pending) BHIP(hipEventSynchronize(S.done));
+  S.pending = false;
+  const double* r = S.h_results + size_t(index) * kResultDoubles;
   if (sums108) std::memcpy(sums108, r + 1, kSumsPerPair * sizeof(double));
   if (dims12)
     for (int i = 0; i < 2 * kMaxScales; ++i) dims12[i] = (int)r[1 + kSumsPerPair + i];
@@ -447,7 +476,7 @@ ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint3
       if ((st = submit(b, npairs, s, false, cfg, mult)) != SSIM2_OK) break;  // warm-up
       hipError_t e = hipEventRecord(e0, s);
       for (uint32_t r = 0; r < reps && e == hipSuccess; ++r)
-        if (enqueue_kernels(b, s, cfg, mult) != SSIM2_OK) e = hipErrorUnknown;
+        if (enqueue_kernels(b, b->slot[b->cur], s, cfg, mult) != SSIM2_OK) e = hipErrorUnknown;
       if (e == hipSuccess) e = hipEventRecord(e1, s);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0;
@@ -468,7 +497,6 @@ ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint3
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->pending = false;
   if (st != SSIM2_OK) return st;
   BHIP(hipStreamSynchronize(s));
   b->tile_cfg = best_cfg;

@@ -178,11 +178,19 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 
 // ---------------------------------------------------------------- pyramid
 
+// sRGB bytes of the pixels one thread converts in a 32x32 tile, for both
+// images, held as loaded (combined only when converted, so that the loads
+// stay in flight meanwhile).
+struct TileBytes {
+  unsigned short lo[2][kPyrTile * kPyrTile / kThreads];  // bytes 0, 1
+  unsigned char hi[2][kPyrTile * kPyrTile / kThreads];   // byte 2
+};
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __restrict__ pairs,
                                                            const int* __restrict__ prefix,
                                                            const int* __restrict__ meta) {
-  constexpr int T0 = kPyrTile;
+  constexpr int T0 = kPyrTile, NP = T0 * T0 / kThreads;
   __shared__ float bufA[6][T0][T0 + 1];
   __shared__ float bufB[6][T0 / 2][T0 / 2 + 1];
   __shared__ float lut_s[256];  // divergent lookups: LDS, not constant memory
@@ -191,24 +199,72 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
   for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut_fast[i];
   lds_cfloat* lut = (lds_cfloat*)lut_s;
   ItemRange items(prefix, npairs, n);
+  // Input bytes of sRGB u8 tiles are fetched one item ahead, so their
+  // latency overlaps the previous tile's pyramid levels.
+  auto fetch = [&](int j, int pj, TileBytes& tb) {
+    const PairDesc& Q = pairs[pj];
+    if (Q.in_kind != kInputSrgbU8) return;
+    const int local = j - prefix[pj];
+    const int ntx = (Q.w + T0 - 1) / T0;
+    const int bx = (local % ntx) * T0, by = (local / ntx) * T0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      // clamped into the image: every lane loads, the conversion skips
+      // pixels outside
+      const int p = tid + k * kThreads;
+      const int gx = min(bx + p % T0, Q.w - 1), gy = min(by + p / T0, Q.h - 1);
+#pragma unroll
+      for (int img = 0; img < 2; ++img) {
+        gin_u8* px = (gin_u8*)(img ? Q.in_dis : Q.in_ref) + gy * (img ? Q.in_dis_pitch : Q.in_ref_pitch) + gx * 3;
+        __builtin_memcpy(&tb.lo[img][k], (const void*)px, 2);
+        tb.hi[img][k] = px[2];
+      }
+    }
+  };
+  TileBytes tb;
+  int pi_next = 0;  // pair of the item fetched ahead (the cursor runs one item early)
+  if (items.first < items.end) {
+    pi_next = items.pair_of(items.first);
+    fetch(items.first, pi_next, tb);
+  }
   for (int idx = items.first; idx < items.end; ++idx) {
-    const int pi = items.pair_of(idx);
+    const int pi = pi_next;
     const PairDesc& P = pairs[pi];
     const int local = idx - prefix[pi];
     const int ntx = (P.w + T0 - 1) / T0;
     const int tx = local % ntx, ty = local / ntx;
     fast_barrier();  // previous item's LDS reads are done
-    for (int p = tid; p < T0 * T0; p += kThreads) {
-      const int lx = p % T0, ly = p / T0;
-      const int gx = tx * T0 + lx, gy = ty * T0 + ly;
-      if (gx >= P.w || gy >= P.h) continue;
+    if (P.in_kind == kInputSrgbU8) {
 #pragma unroll
-      for (int img = 0; img < 2; ++img) {
-        float rgb[3];
-        read_linear(input_of(P, img), gx, gy, lut, rgb);
+      for (int k = 0; k < NP; ++k) {
+        const int p = tid + k * kThreads;
+        const int lx = p % T0, ly = p / T0;
+        if (tx * T0 + lx >= P.w || ty * T0 + ly >= P.h) continue;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = rgb[c];
+        for (int img = 0; img < 2; ++img) {
+          const unsigned lo = tb.lo[img][k];
+          bufA[img * 3 + 0][ly][lx] = lut[lo & 255u];
+          bufA[img * 3 + 1][ly][lx] = lut[lo >> 8];
+          bufA[img * 3 + 2][ly][lx] = lut[tb.hi[img][k]];
+        }
       }
+    } else {
+      for (int p = tid; p < T0 * T0; p += kThreads) {
+        const int lx = p % T0, ly = p / T0;
+        const int gx = tx * T0 + lx, gy = ty * T0 + ly;
+        if (gx >= P.w || gy >= P.h) continue;
+#pragma unroll
+        for (int img = 0; img < 2; ++img) {
+          float rgb[3];
+          read_linear(input_of(P, img), gx, gy, lut, rgb);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = rgb[c];
+        }
+      }
+    }
+    if (idx + 1 < items.end) {
+      pi_next = items.pair_of(idx + 1);
+      fetch(idx + 1, pi_next, tb);
     }
     for (int s = 1; s < P.nscales; ++s) {
       fast_barrier();
