@@ -241,6 +241,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 // the ring. The buffer range check turns reads of rows past the image into 0
 // and drops stores outside the band's rows.
 constexpr int kChunk = 32;                      // columns per staged chunk (8 groups of 4)
+constexpr int kRowPrefetch = 1;                 // chunks of row-pass input in flight per wave
 constexpr int kTileStride = kChunk + 4;         // floats; b128 reads of one row per lane hit distinct banks
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 constexpr unsigned kRsrcWord3 = 0x00020000u;    // raw buffer, 32-bit data format
@@ -259,6 +260,14 @@ __device__ __forceinline__ float4 u2f(u32x4 v) {
 // reordering has to be stopped between the phases.
 __device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory"); }
 
+// s_waitcnt immediate that waits for vmcnt <= N only (gfx9 encoding: vmcnt
+// bits 3:0 and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima).
+template <int N>
+constexpr int vmcnt_only() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  return (N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14);
+}
+
 template <int Q>
 __device__ __forceinline__ float4 row_quantity(float4 x, float4 y) {
   if (Q == 0) return make_float4(x.x * x.x, x.y * x.y, x.z * x.z, x.w * x.w);
@@ -275,11 +284,12 @@ __device__ __forceinline__ float4 row_quantity(float4 x, float4 y) {
 // from x-groups 8k+1..8k+8, i.e. columns 32k+4..32k+35, staged in LDS; the
 // ring keeps the quantities of the last four x-groups (ring[g & 3] = group g).
 //
-// Lanes: each 16-lane DPP row holds five rows of the band as lane triples
-// (lane 15 idle); lane 3t+k runs cosine term k of its row, so a lane issues one
-// recurrence (4 instructions) per column, and the triple's first lane forms
-// (o1 + o3) + o5 with two DPP adds (lanes 3t+1 and 3t+2 shifted down; IEEE
-// addition commutes, so o3 + o1 is o1 + o3).
+// Lanes: each 16-lane row of the wave holds five rows of the band as lane
+// triples (lane 15 idle); lane 3t+k runs cosine term k of its row, so a lane
+// issues one recurrence (4 instructions) per column. Each lane writes its
+// term's outputs to the LDS tile of that term; the coalesced store phase
+// reads the three tiles and forms (o1 + o3) + o5 (cpu.rs's order) once per
+// output, instead of a cross-lane sum per column step.
 //
 // The buffer range check applies to the per-lane offset only, so rows past
 // the image carry an out-of-range per-lane offset and the column is the
@@ -290,7 +300,7 @@ __device__ __forceinline__ vf4 to_vf4(float4 v) { return vf4{v.x, v.y, v.z, v.w}
 template <int Q>
 __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buffer_rsrc_t rb,
                                          __amdgpu_buffer_rsrc_t ro, int w, int pitch, int nrows,
-                                         lds_vf4* tile_a, lds_vf4* tile_b) {
+                                         lds_vf4* tile_a, lds_vf4* tile_b, lds_vf4* tile_c) {
   constexpr bool NA = Q != 1 && Q != 4;
   constexpr bool NB = Q == 1 || Q == 2 || Q == 4;
   constexpr int T4 = kTileStride / 4;        // tile row stride in float4
@@ -299,7 +309,8 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
   const int lane = threadIdx.x;
   const int dl = lane & 15, tr = min(dl / 3, 4), k = dl - 3 * (dl / 3);
   const int r = 5 * (lane >> 4) + tr;        // this lane's row in the band
-  const bool owner = dl < 15 && k == 0;      // the lane that stores its row's outputs
+  const bool active = dl < 15;               // lane 15 of each 16 duplicates lane 12
+  lds_vf4* tile_k = k == 0 ? tile_a : (k == 1 ? tile_b : tile_c);  // this lane's term outputs
   const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
   const float mp = k == 0 ? kMulPrev1 : (k == 1 ? kMulPrev3 : kMulPrev5);
   const u32x4 z = {0u, 0u, 0u, 0u};
@@ -316,24 +327,40 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
   }
   const int glast = (w - 1) / 4;  // last output group
   const int nchunks = glast / 8 + 1;
-  u32x4 qa[NI], qb[NI];
-  auto load_chunk = [&](int kc) {
-#pragma unroll
-    for (int i = 0; i < NI; ++i) {
-      qa[i] = NA ? __builtin_amdgcn_raw_buffer_load_b128(ra, voff[i], kc * 128 + 16, 0) : z;
-      qb[i] = NB ? __builtin_amdgcn_raw_buffer_load_b128(rb, voff[i], kc * 128 + 16, 0) : z;
-    }
-  };
-  load_chunk(0);
   float4 ring[4];
   {
-    // x-group 0, then the steps n = -4 .. -1 (no outputs): right inputs
-    // x-group 0, left inputs the zeros of x-groups -3 and -2.
+    // x-group 0 (loaded before the chunks: it is needed first), then the
+    // steps n = -4 .. -1 (no outputs): right inputs x-group 0, left inputs
+    // the zeros of x-groups -3 and -2.
     const unsigned vrow = r < nrows ? (unsigned)(r * pitch * 4) : kOffNone;
     const float4 a = NA ? u2f(__builtin_amdgcn_raw_buffer_load_b128(ra, vrow, 0, 0)) : z4;
     const float4 b = NB ? u2f(__builtin_amdgcn_raw_buffer_load_b128(rb, vrow, 0, 0)) : z4;
     ring[0] = row_quantity<Q>(a, b);
     ring[1] = ring[2] = ring[3] = z4;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  // kRowPrefetch chunks of input in flight: one wave per SIMD cannot hide an
+  // HBM round trip behind one chunk of scan. Each buffer keeps its own
+  // registers (the chunk loop is unrolled by kRowPrefetch), since copying a
+  // register that a load is still filling would wait for the load.
+  u32x4 qa[kRowPrefetch][NI], qb[kRowPrefetch][NI];
+  auto load_chunk = [&](int kc, u32x4(&da)[NI], u32x4(&db)[NI]) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      da[i] = NA ? __builtin_amdgcn_raw_buffer_load_b128(ra, voff[i], kc * 128 + 16, 0) : z;
+      db[i] = NB ? __builtin_amdgcn_raw_buffer_load_b128(rb, voff[i], kc * 128 + 16, 0) : z;
+    }
+  };
+  // Every load and chunk is unconditional, so the loads' wait counts stay
+  // exact: the chunk count is padded to a multiple of kRowPrefetch, loads
+  // past the last chunk re-read it, and the padded chunks' stores carry an
+  // out-of-range offset (8 kc > glast), so they write nothing.
+  const int nchunks_pad = (nchunks + kRowPrefetch - 1) / kRowPrefetch * kRowPrefetch;
+  // in chunk order (the first chunk's data is needed first)
+#pragma unroll
+  for (int d = 0; d < kRowPrefetch; ++d) {
+    load_chunk(min(d, nchunks - 1), qa[d], qb[d]);
+    __builtin_amdgcn_sched_barrier(0);
   }
   float p = 0.f, q = 0.f;
   // one step of this lane's term (cpu.rs:992-1010 for term k)
@@ -346,26 +373,24 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
     p = o;
     return o;
   };
-  // (o1 + o3) + o5 on the triple's first lane: lanes 3t+1 and 3t+2 shifted
-  // down within the 16-lane row (row_shl:1 / :2, zero past the row's end,
-  // which only lanes that own no row see)
-  auto row_sum = [](float o) {
-    const float s1 = __builtin_amdgcn_update_dpp(0.0f, o, 0x101, 0xF, 0xF, true) + o;
-    return __builtin_amdgcn_update_dpp(0.0f, o, 0x102, 0xF, 0xF, true) + s1;
-  };
   step(0.f, ring[0].x);
   step(0.f, ring[0].y);
   step(0.f, ring[0].z);
   step(0.f, ring[0].w);
-  for (int kc = 0; kc < nchunks; ++kc) {
+  auto chunk = [&](int kc, u32x4(&da)[NI], u32x4(&db)[NI]) {
+    // This chunk's loads are done once at most the later chunks' loads are
+    // outstanding: loads complete in issue order, but stores (some of them
+    // wholly out of range) may complete before earlier loads, so the wait
+    // must not count on the stores issued in between.
+    __builtin_amdgcn_s_waitcnt(vmcnt_only<(kRowPrefetch - 1) * NI * (int(NA) + int(NB))>());
     // stage x columns 32k+4 .. 32k+35 of the band's rows into the padded tiles
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-      if (NA) tile_a[eidx[i]] = to_vf4(u2f(qa[i]));
-      if (NB) tile_b[eidx[i]] = to_vf4(u2f(qb[i]));
+      if (NA) tile_a[eidx[i]] = to_vf4(u2f(da[i]));
+      if (NB) tile_b[eidx[i]] = to_vf4(u2f(db[i]));
     }
     wave_lds_order();
-    load_chunk(kc + 1);  // past the last chunk: 0 or columns no output uses
+    load_chunk(min(kc + kRowPrefetch, nchunks - 1), da, db);
     float4 xa[8], xb[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -380,40 +405,56 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
     for (int j = 0; j < 8; ++j) {
       ring[(j + 1) & 3] = row_quantity<Q>(xa[j], xb[j]);  // x-group 8k+j+1
       const float4 rr = ring[(j + 1) & 3], m2 = ring[(j + 2) & 3], m1 = ring[(j + 3) & 3];
-      out[j].x = row_sum(step(m2.z, rr.x));
-      out[j].y = row_sum(step(m2.w, rr.y));
-      out[j].z = row_sum(step(m1.x, rr.z));
-      out[j].w = row_sum(step(m1.y, rr.w));
+      out[j].x = step(m2.z, rr.x);
+      out[j].y = step(m2.w, rr.y);
+      out[j].z = step(m1.x, rr.z);
+      out[j].w = step(m1.y, rr.w);
     }
-    // back through the tile: the owners' rows in, coalesced rows out; segments
-    // past the last output group are dropped with an out-of-range offset
+    // back through the tiles: every term's rows in, coalesced rows out with
+    // the terms summed; segments past the last output group are dropped with
+    // an out-of-range offset
     wave_lds_order();
-    if (owner) {
+    if (active) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) tile_a[r * T4 + j] = to_vf4(out[j]);
+      for (int j = 0; j < 8; ++j) tile_k[r * T4 + j] = to_vf4(out[j]);
     }
     wave_lds_order();
     float4 v[NI];
 #pragma unroll
-    for (int i = 0; i < NI; ++i) v[i] = to_f4(tile_a[eidx[i]]);
+    for (int i = 0; i < NI; ++i) {
+      const float4 o1 = to_f4(tile_a[eidx[i]]), o3 = to_f4(tile_b[eidx[i]]), o5 = to_f4(tile_c[eidx[i]]);
+      v[i] = make_float4((o1.x + o3.x) + o5.x, (o1.y + o3.y) + o5.y, (o1.z + o3.z) + o5.z, (o1.w + o3.w) + o5.w);
+    }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
       const bool seg_ok = 8 * kc + ((64 * i + lane) & 7) <= glast;
       const u32x4 u = {__float_as_uint(v[i].x), __float_as_uint(v[i].y), __float_as_uint(v[i].z),
                        __float_as_uint(v[i].w)};
-      __builtin_amdgcn_raw_buffer_store_b128(u, ro, seg_ok ? voff[i] : kOffNone, kc * 128, 0);
+      // Column offset in the per-lane offset and an immediate scalar offset of
+      // 0: a 16-byte buffer store reads its data registers after issue, and
+      // the compiler only guards them against the next instructions (wait
+      // states) when the scalar offset is not a register. The range check
+      // still drops kOffNone + kc*128 (< 2^32).
+      __builtin_amdgcn_raw_buffer_store_b128(u, ro, (seg_ok ? voff[i] : kOffNone) + (unsigned)(kc * 128), 0, 0);
     }
     wave_lds_order();
+  };
+  for (int kc = 0; kc < nchunks_pad; kc += kRowPrefetch) {
+#pragma unroll
+    for (int d = 0; d < kRowPrefetch; ++d) chunk(kc + d, qa[d], qb[d]);
   }
 }
 
 template <int SIDES>
 __global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
                                              const WorkItem* __restrict__ items) {
-  __shared__ float4 tiles[2][kRowsPerWave * kTileStride / 4];
+  // input tiles of the two images, reused with a third tile for the three
+  // terms' outputs
+  __shared__ float4 tiles[3][kRowsPerWave * kTileStride / 4];
   lds_vf4* tile_a = (lds_vf4*)tiles[0];
   lds_vf4* tile_b = (lds_vf4*)tiles[1];
+  lds_vf4* tile_c = (lds_vf4*)tiles[2];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b / kNumQ, q = it.b % kNumQ;
@@ -427,11 +468,11 @@ __global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
   const __amdgpu_buffer_rsrc_t ro =
       buffer_rsrc(P.arena + S.hrow + (long long)(c * kNumQ + q) * S.plane + off, bytes);
   switch (q) {
-    case 0: row_scan<0>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
-    case 1: row_scan<1>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
-    case 2: row_scan<2>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
-    case 3: row_scan<3>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
-    default: row_scan<4>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b); break;
+    case 0: row_scan<0>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
+    case 1: row_scan<1>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
+    case 2: row_scan<2>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
+    case 3: row_scan<3>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
+    default: row_scan<4>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
   }
 }
 

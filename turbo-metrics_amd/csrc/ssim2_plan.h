@@ -2,6 +2,7 @@
 // single-pair context and the batch engine).
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "ssim2_internal.h"
@@ -85,14 +86,27 @@ inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPla
   p.n_pyr = (int)p.items.size();
   // Row pass: one wave per (scale, channel*5+quantity, band of kRowsPerWave
   // rows), largest scale first; one list per Sides (its quantities only).
+  // The waves of one band read the same two XYB rows, so they are placed on
+  // one XCD: workgroups go to the 8 XCDs round-robin by id, and in each group
+  // of 8 bands the quantity waves of band j get ids = j (mod 8), so they
+  // share that XCD's L2.
   static const int kQ[3][5] = {{0, 1, 2, 3, 4}, {0, 3, -1, -1, -1}, {1, 2, 4, -1, -1}};
   for (int sd = 0; sd < 3; ++sd) {
     p.off_rows[sd] = (int)p.items.size();
+    int nq = 0;
+    while (nq < 5 && kQ[sd][nq] >= 0) ++nq;
+    std::vector<WorkItem> bands;  // (scale, channel, band), largest scale first
     for (int s = 0; s < d.nscales; ++s)
       for (int c = 0; c < 3; ++c)
-        for (int qi = 0; qi < 5 && kQ[sd][qi] >= 0; ++qi)
-          for (int b = 0; b < (sh[s] + kRowsPerWave - 1) / kRowsPerWave; ++b)
-            p.items.push_back({pair, s, c * kNumQ + kQ[sd][qi], b});
+        for (int b = 0; b < (sh[s] + kRowsPerWave - 1) / kRowsPerWave; ++b) bands.push_back({pair, s, c, b});
+    for (size_t g = 0; g < bands.size(); g += 8) {
+      const size_t n = std::min<size_t>(8, bands.size() - g);
+      for (int qi = 0; qi < nq; ++qi)
+        for (size_t j = 0; j < n; ++j) {
+          const WorkItem& B = bands[g + j];
+          p.items.push_back({pair, B.a, B.b * kNumQ + kQ[sd][qi], B.c});
+        }
+    }
     p.n_rows[sd] = (int)p.items.size() - p.off_rows[sd];
   }
   // Column pass + maps: (scale, channel, 16-column strip).
