@@ -483,29 +483,38 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One block per (scale, channel, strip of 32 columns); each wave scans two of
-// the blurred quantities (lanes 0-31 one, lanes 32-63 the next), so a block
-// has ceil(NQ / 2) waves, at most one per SIMD. Lane x of quantity slot j
-// scans column x with the three cosine terms of cpu.rs:1054-1115 (column
-// form) in registers and forms (o1 + o3) + o5 itself. Rows go in chunks of
-// kColChunk: the bottom inputs (row n+4 for step n) of the next chunk are
-// loaded while the current one is scanned, and the top inputs (row n-6) are
-// earlier bottom inputs kept in registers. Loads are buffer loads with the row
-// as a scalar offset, clamped to the 4 zero rows under every plane, so no
-// lane computes a row address or a mask. Each chunk's outputs go to a
-// double-buffered LDS tile; after one barrier all threads evaluate the SSIM
-// and edge-difference maps of the chunk's pixels with f64 L1/L4 sums
-// (cpu.rs:581-683).
+// One block per (scale, channel, strip of 32 columns) with two kinds of waves.
+// Scan waves (ceil(NQ / 2)): each scans two of the blurred quantities (lanes
+// 0-31 one, lanes 32-63 the next); lane x of quantity slot j scans column x
+// with the three cosine terms of cpu.rs:1054-1115 (column form) in registers
+// and forms (o1 + o3) + o5 itself. Rows go in chunks of kColChunk: the bottom
+// inputs (row n+4 for step n) of the next chunk are loaded while the current
+// one is scanned, and the top inputs (row n-6) are earlier bottom inputs kept
+// in registers. Loads are buffer loads with the row as a scalar offset,
+// clamped to the 4 zero rows under every plane, so no lane computes a row
+// address or a mask. Map waves (twice as many) evaluate the SSIM and
+// edge-difference maps (cpu.rs:581-683) with f64 L1/L4 sums. The two run one
+// chunk apart on a double-buffered LDS tile, separated by one barrier per
+// phase: in phase k the scan waves produce chunk k while the map waves
+// consume chunk k-1, so the maps are off the column scan's sequential path.
+// Waves of a column-pass block: scan waves (two quantities each), then map waves.
 template <int SIDES>
-__global__ __launch_bounds__(64 * ((SideTraits<SIDES>::NQ + 1) / 2)) void k_cols(
+struct ColWaves {
+  static constexpr int scan = (SideTraits<SIDES>::NQ + 1) / 2;
+  static constexpr int map = 2 * scan;  // 1x: 58.8 us at 1080p, 2x: 53.7, 3x: 52.3
+  static constexpr int threads = 64 * (scan + map);
+};
+
+template <int SIDES>
+__global__ __launch_bounds__(ColWaves<SIDES>::threads) void k_cols(
     const PairDesc* __restrict__ pairs, const WorkItem* __restrict__ items) {
   constexpr int NQ = SideTraits<SIDES>::NQ;
-  constexpr int NW = (NQ + 1) / 2;
-  constexpr int RY = kColChunk, CB = kColBlock, NT = 64 * NW;
+  constexpr int NW = ColWaves<SIDES>::scan, MW = ColWaves<SIDES>::map;
+  constexpr int RY = kColChunk, CB = kColBlock, NT = 64 * MW;  // NT: map threads
   static_assert(CB == 32, "two quantities of 32 columns per wave");
   static_assert(RY >= 2 * kRadius, "top inputs come from the previous chunk");
   __shared__ float O[2][NQ][RY][CB];
-  __shared__ double red[NW][kNumSums];
+  __shared__ double red[MW][kNumSums];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b, xb = it.c * CB;
@@ -513,37 +522,82 @@ __global__ __launch_bounds__(64 * ((SideTraits<SIDES>::NQ + 1) / 2)) void k_cols
   const int w = S.w, h = S.h, pitch = S.pitch;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int x = lane & 31;
-  const int slot = min(2 * wv + (lane >> 5), NQ - 1);  // lanes past NQ rescan the last one, unused
-  const int q0 = SideTraits<SIDES>::q(2 * wv), q = SideTraits<SIDES>::q(slot);
-  // the wave's first plane as the buffer base; the second one by its distance
-  // (at most 3 planes, < 2^32 bytes); rows are clamped, so no range check is needed
-  const __amdgpu_buffer_rsrc_t rq =
-      __builtin_amdgcn_make_buffer_rsrc(P.arena + S.hrow + (long long)(c * kNumQ + q0) * S.plane, 0, -1, kRsrcWord3);
-  const unsigned vcol = (unsigned)((long long)(q - q0) * S.plane * 4 + (xb + x) * 4);
-  const int row_bytes = pitch * 4;
-  auto load_chunk = [&](int y0, float* B) {
+  const int nch = (h + kRadius - 1 + RY - 1) / RY;  // chunks: rows -kRadius+1 .. h-1
+  const int y00 = -kRadius + 1;
+
+  if (wv < NW) {
+    // ---------------------------------------------------------------- scan waves
+    const int x = lane & 31;
+    const int slot = min(2 * wv + (lane >> 5), NQ - 1);  // lanes past NQ rescan the last one, unused
+    const int q0 = SideTraits<SIDES>::q(2 * wv), q = SideTraits<SIDES>::q(slot);
+    // the wave's first plane as the buffer base; the second one by its
+    // distance (at most 3 planes, < 2^32 bytes); rows are clamped, so no range
+    // check is needed
+    const __amdgpu_buffer_rsrc_t rq =
+        __builtin_amdgcn_make_buffer_rsrc(P.arena + S.hrow + (long long)(c * kNumQ + q0) * S.plane, 0, -1, kRsrcWord3);
+    const unsigned vcol = (unsigned)((long long)(q - q0) * S.plane * 4 + (xb + x) * 4);
+    const int row_bytes = pitch * 4;
+    auto load_chunk = [&](int y0, float* B) {
 #pragma unroll
-    for (int i = 0; i < RY; ++i) {
-      const int r = min(y0 + i + kRadius - 1, h + kPadRows - 1);  // rows h .. h+3 are zero
-      B[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rq, vcol, r * row_bytes, 0));
+      for (int i = 0; i < RY; ++i) {
+        const int r = min(y0 + i + kRadius - 1, h + kPadRows - 1);  // rows h .. h+3 are zero
+        B[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rq, vcol, r * row_bytes, 0));
+      }
+    };
+    float Bprev[RY], Bcur[RY], Bnext[RY];
+#pragma unroll
+    for (int i = 0; i < RY; ++i) Bprev[i] = 0.0f;
+    load_chunk(y00, Bcur);
+    float p1 = 0.f, p3 = 0.f, p5 = 0.f, q1 = 0.f, q3 = 0.f, q5 = 0.f;
+    for (int k = 0; k <= nch; ++k) {
+      if (k < nch) {
+        const int y0 = y00 + k * RY;
+        load_chunk(y0 + RY, Bnext);
+        const int buf = k & 1;
+#pragma unroll
+        for (int i = 0; i < RY; ++i) {
+          // top input row y0+i-6 = bottom input of step i-10
+          const float top = i >= 2 * kRadius ? Bcur[i - 2 * kRadius] : Bprev[i + RY - 2 * kRadius];
+          const float sum = top + Bcur[i];
+          const float t1 = fmaf(p1, kVertMulPrev1, q1), t3 = fmaf(p3, kVertMulPrev3, q3),
+                      t5 = fmaf(p5, kVertMulPrev5, q5);
+          const float o1 = fmaf(sum, kMulIn1, -t1), o3 = fmaf(sum, kMulIn3, -t3), o5 = fmaf(sum, kMulIn5, -t5);
+          q1 = p1;
+          q3 = p3;
+          q5 = p5;
+          p1 = o1;
+          p3 = o3;
+          p5 = o5;
+          O[buf][slot][i][x] = (o1 + o3) + o5;
+        }
+#pragma unroll
+        for (int i = 0; i < RY; ++i) {
+          Bprev[i] = Bcur[i];
+          Bcur[i] = Bnext[i];
+        }
+      }
+      lds_barrier();
     }
-  };
+    if (SIDES != kSidesRef) lds_barrier();  // the map waves' reduction barrier
+    return;
+  }
+
+  // ------------------------------------------------------------------ map waves
+  const int mt = tid - 64 * NW;  // thread index among the map waves
   gcfloat* srcp = gbl(xyb_plane(P, s, 0, c, SIDES));
   gcfloat* disp = gbl(xyb_plane(P, s, 1, c, SIDES));
   gfloat* cache_s11 = gbl(P.arena + S.cache + (long long)(3 + c) * S.plane);
   gfloat* cache_mu1 = gbl(P.arena + S.cache + (long long)(6 + c) * S.plane);
-
   // Map inputs (source and distorted XYB, and the cached reference statistics
   // when only the distorted side runs) of this thread's pixels, loaded one
-  // chunk ahead: the maps phase would otherwise wait out an HBM latency per chunk.
+  // chunk ahead: the maps would otherwise wait out an HBM latency per chunk.
   constexpr int NM = (RY * CB + NT - 1) / NT;  // map pixels per thread per chunk
   constexpr int NV = SIDES == kSidesDis ? 4 : (SIDES == kSidesBoth ? 2 : 0);
   float mv[NM][NV > 0 ? NV : 1], mv_next[NM][NV > 0 ? NV : 1];
   auto load_maps = [&](int y0c, float (*dst)[NV > 0 ? NV : 1]) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      const int px = tid + NT * m;
+      const int px = mt + NT * m;
       const int n = min(max(y0c + px / CB, 0), h - 1), col = min(xb + px % CB, w - 1);
       const long long off = (long long)n * pitch + col;
       if (NV >= 2) {
@@ -556,103 +610,78 @@ __global__ __launch_bounds__(64 * ((SideTraits<SIDES>::NQ + 1) / 2)) void k_cols
       }
     }
   };
-
-  float Bprev[RY], Bcur[RY], Bnext[RY];
-#pragma unroll
-  for (int i = 0; i < RY; ++i) Bprev[i] = 0.0f;
-  int y0 = -kRadius + 1;
-  load_chunk(y0, Bcur);
-  load_maps(y0, mv);
-  float p1 = 0.f, p3 = 0.f, p5 = 0.f, q1 = 0.f, q3 = 0.f, q5 = 0.f;
+  load_maps(y00, mv);
   double acc[kNumSums] = {0, 0, 0, 0, 0, 0};
-  for (int ci = 0; y0 < h; ++ci, y0 += RY) {
-    load_chunk(y0 + RY, Bnext);
-    const int buf = ci & 1;
-    load_maps(y0 + RY, mv_next);
+  for (int k = 0; k <= nch; ++k) {
+    if (k >= 1) {
+      // chunk k-1, produced in the previous phase
+      const int y0 = y00 + (k - 1) * RY, buf = (k - 1) & 1;
+      load_maps(y0 + RY, mv_next);
 #pragma unroll
-    for (int i = 0; i < RY; ++i) {
-      // top input row y0+i-6 = bottom input of step i-10
-      const float top = i >= 2 * kRadius ? Bcur[i - 2 * kRadius] : Bprev[i + RY - 2 * kRadius];
-      const float sum = top + Bcur[i];
-      const float t1 = fmaf(p1, kVertMulPrev1, q1), t3 = fmaf(p3, kVertMulPrev3, q3),
-                  t5 = fmaf(p5, kVertMulPrev5, q5);
-      const float o1 = fmaf(sum, kMulIn1, -t1), o3 = fmaf(sum, kMulIn3, -t3), o5 = fmaf(sum, kMulIn5, -t5);
-      q1 = p1;
-      q3 = p3;
-      q5 = p5;
-      p1 = o1;
-      p3 = o3;
-      p5 = o5;
-      O[buf][slot][i][x] = (o1 + o3) + o5;
-    }
-    lds_barrier();
+      for (int m = 0; m < NM; ++m) {
+        const int px = mt + NT * m;
+        const int i = px / CB, col = xb + px % CB, n = y0 + i;
+        if ((RY * CB) % NT == 0 || px < RY * CB) {
+          if (n >= 0 && n < h && col < w) {
+            float bq[NQ];
 #pragma unroll
-    for (int m = 0; m < (RY * CB + NT - 1) / NT; ++m) {
-      const int px = tid + NT * m;
-      const int i = px / CB, col = xb + px % CB, n = y0 + i;
-      if ((RY * CB) % NT == 0 || px < RY * CB) {
-        if (n >= 0 && n < h && col < w) {
-          float bq[NQ];
-#pragma unroll
-          for (int q2 = 0; q2 < NQ; ++q2) bq[q2] = O[buf][q2][i][px % CB];
-          const long long off = (long long)n * pitch + col;
-          if (SIDES == kSidesRef) {
-            cache_s11[off] = bq[0];
-            cache_mu1[off] = bq[1];
-          } else {
-            const float* v = mv[m];
-            float s11, s22, s12, mu1, mu2;
-            if (SIDES == kSidesBoth) {
-              s11 = bq[0];
-              s22 = bq[1];
-              s12 = bq[2];
-              mu1 = bq[3];
-              mu2 = bq[4];
+            for (int q2 = 0; q2 < NQ; ++q2) bq[q2] = O[buf][q2][i][px % CB];
+            const long long off = (long long)n * pitch + col;
+            if (SIDES == kSidesRef) {
+              cache_s11[off] = bq[0];
+              cache_mu1[off] = bq[1];
             } else {
-              s11 = v[2];
-              mu1 = v[3];
-              s22 = bq[0];
-              s12 = bq[1];
-              mu2 = bq[2];
+              const float* v = mv[m];
+              float s11, s22, s12, mu1, mu2;
+              if (SIDES == kSidesBoth) {
+                s11 = bq[0];
+                s22 = bq[1];
+                s12 = bq[2];
+                mu1 = bq[3];
+                mu2 = bq[4];
+              } else {
+                s11 = v[2];
+                mu1 = v[3];
+                s22 = bq[0];
+                s12 = bq[1];
+                mu2 = bq[2];
+              }
+              const double d = ssim_error(mu1, mu2, s11, s22, s12);
+              const double d2 = d * d;
+              acc[0] += d;
+              acc[1] += d2 * d2;
+              const double d1 = edge_ratio(v[0], mu1, v[1], mu2);
+              const double art = d1 > 0.0 ? d1 : 0.0;
+              const double det = -d1 > 0.0 ? -d1 : 0.0;
+              const double a2 = art * art, e2 = det * det;
+              acc[2] += art;
+              acc[3] += a2 * a2;
+              acc[4] += det;
+              acc[5] += e2 * e2;
             }
-            const double d = ssim_error(mu1, mu2, s11, s22, s12);
-            const double d2 = d * d;
-            acc[0] += d;
-            acc[1] += d2 * d2;
-            const double d1 = edge_ratio(v[0], mu1, v[1], mu2);
-            const double art = d1 > 0.0 ? d1 : 0.0;
-            const double det = -d1 > 0.0 ? -d1 : 0.0;
-            const double a2 = art * art, e2 = det * det;
-            acc[2] += art;
-            acc[3] += a2 * a2;
-            acc[4] += det;
-            acc[5] += e2 * e2;
           }
         }
       }
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+#pragma unroll
+        for (int q2 = 0; q2 < (NV > 0 ? NV : 1); ++q2) mv[m][q2] = mv_next[m][q2];
     }
-#pragma unroll
-    for (int i = 0; i < RY; ++i) {
-      Bprev[i] = Bcur[i];
-      Bcur[i] = Bnext[i];
-    }
-#pragma unroll
-    for (int m = 0; m < NM; ++m)
-#pragma unroll
-      for (int k = 0; k < (NV > 0 ? NV : 1); ++k) mv[m][k] = mv_next[m][k];
+    lds_barrier();
   }
   if (SIDES == kSidesRef) return;
-  // Block reduction in a fixed order: wave shuffles, then waves 0..NW-1 in sequence.
+  // Map-wave reduction in a fixed order: wave shuffles, then the map waves in
+  // sequence (the scan waves wait at the same barrier).
 #pragma unroll
   for (int k = 0; k < kNumSums; ++k) {
     const double v = wave_sum(acc[k]);
-    if (lane == 0) red[wv][k] = v;
+    if (lane == 0) red[wv - NW][k] = v;
   }
   lds_barrier();
-  if (tid < kNumSums) {
+  if (mt < kNumSums) {
     double v = 0.0;
-    for (int k = 0; k < NW; ++k) v += red[k][tid];
-    P.partials[(long long)(S.part + c * S.colblocks + it.c) * kNumSums + tid] = v;
+    for (int k = 0; k < MW; ++k) v += red[k][mt];
+    P.partials[(long long)(S.part + c * S.colblocks + it.c) * kNumSums + mt] = v;
   }
 }
 
@@ -723,13 +752,13 @@ hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems,
                        hipStream_t stream) {
   if (nitems <= 0) return hipSuccess;
   if (sides == kSidesBoth)
-    hipLaunchKernelGGL(k_cols<kSidesBoth>, dim3(nitems), dim3(64 * ((SideTraits<kSidesBoth>::NQ + 1) / 2)), 0, stream, pairs,
+    hipLaunchKernelGGL(k_cols<kSidesBoth>, dim3(nitems), dim3(ColWaves<kSidesBoth>::threads), 0, stream, pairs,
                        items);
   else if (sides == kSidesRef)
-    hipLaunchKernelGGL(k_cols<kSidesRef>, dim3(nitems), dim3(64 * ((SideTraits<kSidesRef>::NQ + 1) / 2)), 0, stream, pairs,
+    hipLaunchKernelGGL(k_cols<kSidesRef>, dim3(nitems), dim3(ColWaves<kSidesRef>::threads), 0, stream, pairs,
                        items);
   else
-    hipLaunchKernelGGL(k_cols<kSidesDis>, dim3(nitems), dim3(64 * ((SideTraits<kSidesDis>::NQ + 1) / 2)), 0, stream, pairs,
+    hipLaunchKernelGGL(k_cols<kSidesDis>, dim3(nitems), dim3(ColWaves<kSidesDis>::threads), 0, stream, pairs,
                        items);
   return hipGetLastError();
 }
