@@ -483,6 +483,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Waves of a column-pass block: scan waves (two quantities each), then map waves.
+template <int SIDES>
+struct ColWaves {
+  static constexpr int scan = (SideTraits<SIDES>::NQ + 1) / 2;
+  static constexpr int map = 2 * scan;  // 1x: 58.8 us at 1080p, 2x: 53.7, 3x: 52.3
+  static constexpr int threads = 64 * (scan + map);
+};
+
 // One block per (scale, channel, strip of 32 columns) with two kinds of waves.
 // Scan waves (ceil(NQ / 2)): each scans two of the blurred quantities (lanes
 // 0-31 one, lanes 32-63 the next); lane x of quantity slot j scans column x
@@ -497,14 +505,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 // chunk apart on a double-buffered LDS tile, separated by one barrier per
 // phase: in phase k the scan waves produce chunk k while the map waves
 // consume chunk k-1, so the maps are off the column scan's sequential path.
-// Waves of a column-pass block: scan waves (two quantities each), then map waves.
-template <int SIDES>
-struct ColWaves {
-  static constexpr int scan = (SideTraits<SIDES>::NQ + 1) / 2;
-  static constexpr int map = 2 * scan;  // 1x: 58.8 us at 1080p, 2x: 53.7, 3x: 52.3
-  static constexpr int threads = 64 * (scan + map);
-};
-
 template <int SIDES>
 __global__ __launch_bounds__(ColWaves<SIDES>::threads) void k_cols(
     const PairDesc* __restrict__ pairs, const WorkItem* __restrict__ items) {
