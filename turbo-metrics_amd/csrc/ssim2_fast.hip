@@ -28,7 +28,7 @@
 //   k_fast_tile     one TXxTY output tile of one scale of one pair per item:
 //                   XYB tile + 4 px halo -> products -> horizontal FIR ->
 //                   vertical FIR -> maps -> f64 block sums.
-//   finalisation    k_reduce_groups + k_score (ssim2_finalize.hip).
+//   finalisation    k_finalize, one block per pair (ssim2_finalize.hip).
 #include <type_traits>
 
 #include "ssim2_input.h"

@@ -2,13 +2,14 @@
 // per-(scale, channel) reduction of the partial-sum records, then the
 // 108-feature score (crates/ssimulacra2-cuda/examples/cpu.rs:633-634, 676-679
 // for the averages, 840-868 for the weighted sum and transform), in f64 on the
-// device. Two launches:
-//   k_reduce_groups  one block per (pair, scale*3 + channel) unit, persistent:
-//                    256 threads stride over the records, then a fixed-order
-//                    tree (wave shuffles, waves in sequence); the order does
-//                    not depend on which other pairs share the launch.
-//   k_score          one wave per pair: the 108 features in parallel, then the
-//                    weighted sum in cpu.rs's order on one lane.
+// device, in one launch. One block per pair (persistent over pairs): its waves
+// take the 18 (scale, channel) groups in turn, each lane striding over the
+// group's records and the wave summing its lanes with a fixed shuffle tree;
+// after a block barrier the 108 features are formed in parallel and one lane
+// forms the weighted sum in cpu.rs's order. Every order is fixed and depends on
+// the pair alone, so a pair scores the same in any batch. (No cross-block
+// step: a device-scope fence inside a kernel writes back and invalidates each
+// XCD's L2.)
 #include <algorithm>
 
 #include "ssim2_internal.h"
@@ -35,60 +36,49 @@ __device__ __forceinline__ void lds_sync() {
 
 }  // namespace
 
-__global__ __launch_bounds__(kThreads) void k_reduce_groups(const PairDesc* __restrict__ pairs,
-                                                            const int* __restrict__ meta, int npairs_static) {
-  __shared__ double red[kThreads / 64][kNumSums];
-  const int npairs = meta ? meta[0] : npairs_static;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int u = blockIdx.x; u < npairs * kGroups; u += gridDim.x) {
-    const PairDesc& P = pairs[u / kGroups];
-    const int g = u % kGroups, s = g / 3, c = g % 3;
-    double v[kNumSums] = {0, 0, 0, 0, 0, 0};
-    if (s < P.nscales) {
-      const ScaleDesc& S = P.sc[s];
-      const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums;
-      for (int b = t; b < S.colblocks; b += kThreads)
-#pragma unroll
-        for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
-    }
-    lds_sync();  // previous unit's red[] reads are done
-#pragma unroll
-    for (int q = 0; q < kNumSums; ++q) {
-      const double r = wave_sum_f(v[q]);
-      if (lane == 0) red[wv][q] = r;
-    }
-    lds_sync();
-    if (t < kNumSums) {
-      double r = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) r += red[w][t];
-      P.result[1 + g * kNumSums + t] = r;
-    }
-    if (g == 0 && t < 2 * kMaxScales) {
-      const int ss = t / 2;
-      P.result[1 + kSumsPerPair + t] = ss < P.nscales ? (double)(t & 1 ? P.sc[ss].h : P.sc[ss].w) : 0.0;
-    }
-  }
-}
+constexpr int kFinThreads = 512;
 
-__global__ __launch_bounds__(64) void k_score(const PairDesc* __restrict__ pairs, const int* __restrict__ meta,
-                                              int npairs_static) {
+__global__ __launch_bounds__(kFinThreads) void k_finalize(const PairDesc* __restrict__ pairs,
+                                                          const int* __restrict__ meta, int npairs_static) {
+  __shared__ double sums[kGroups][kNumSums];
   __shared__ double feat[kSumsPerPair];
   const int npairs = meta ? meta[0] : npairs_static;
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   for (int pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
     const PairDesc& P = pairs[pi];
+    for (int g = wv; g < kGroups; g += kFinThreads / 64) {
+      const int s = g / 3, c = g % 3;
+      double v[kNumSums] = {0, 0, 0, 0, 0, 0};
+      if (s < P.nscales) {
+        const ScaleDesc& S = P.sc[s];
+        const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums;
+        for (int b = lane; b < S.colblocks; b += 64)
+#pragma unroll
+          for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
+      }
+#pragma unroll
+      for (int q = 0; q < kNumSums; ++q) {
+        const double r = wave_sum_f(v[q]);
+        if (lane == 0) sums[g][q] = r;
+      }
+    }
+    lds_sync();
+    if (t < kSumsPerPair) P.result[1 + t] = sums[t / kNumSums][t % kNumSums];
+    if (t >= kSumsPerPair && t < kSumsPerPair + 2 * kMaxScales) {
+      const int k = t - kSumsPerPair, ss = k / 2;
+      P.result[1 + kSumsPerPair + k] = ss < P.nscales ? (double)(k & 1 ? P.sc[ss].h : P.sc[ss].w) : 0.0;
+    }
     const int ns = P.nscales, nf = 3 * ns * 6;
     // feature k in cpu.rs order: channel, present scale, {L1 of the three maps, L4 of the three}
-    for (int k = lane; k < nf; k += 64) {
-      const int c = k / (ns * 6), r = k % (ns * 6), s = r / 6, j = r % 6;
-      const double opp = 1.0 / (double)((long long)P.sc[s].w * P.sc[s].h);
-      const double* q = P.result + 1 + (s * 3 + c) * kNumSums;
-      feat[k] = j < 3 ? fabs(opp * q[2 * j]) : fabs(sqrt(sqrt(opp * q[2 * (j - 3) + 1])));
+    if (t < nf) {
+      const int fc = t / (ns * 6), r = t % (ns * 6), fs = r / 6, j = r % 6;
+      const double opp = 1.0 / (double)((long long)P.sc[fs].w * P.sc[fs].h);
+      const double* q = sums[fs * 3 + fc];
+      feat[t] = j < 3 ? fabs(opp * q[2 * j]) : fabs(sqrt(sqrt(opp * q[2 * (j - 3) + 1])));
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    if (lane == 0) {
+    lds_sync();
+    if (t == 0) {
       double ssim = 0.0;
       for (int k = 0; k < nf; ++k) ssim = fma(c_weight_fin[k], feat[k], ssim);
       ssim *= 0.9562382616834844;
@@ -97,19 +87,15 @@ __global__ __launch_bounds__(64) void k_score(const PairDesc* __restrict__ pairs
       ssim = ssim > 0.0 ? fma(pow(ssim, 0.6276336467831387), -10.0, 100.0) : 100.0;
       P.result[0] = ssim;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    lds_sync();  // sums[] and feat[] are free for the next pair
   }
 }
 
 hipError_t launch_finalize_pairs(const PairDesc* pairs, const int* meta, int npairs_static, int grid_units,
                                  int max_pairs, hipStream_t stream) {
-  hipLaunchKernelGGL(k_reduce_groups, dim3(grid_units), dim3(kThreads), 0, stream, pairs, meta, npairs_static);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_score, dim3(std::max(1, std::min(max_pairs, 1024))), dim3(64), 0, stream, pairs, meta,
-                     npairs_static);
+  (void)grid_units;
+  const int grid = std::max(1, std::min(max_pairs, 1024));
+  hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(kFinThreads), 0, stream, pairs, meta, npairs_static);
   return hipGetLastError();
 }
 
