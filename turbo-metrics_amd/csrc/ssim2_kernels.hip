@@ -144,15 +144,47 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
   }
   lds_cfloat* lut = (lds_cfloat*)lut_s;
 
-  // Scale 0: input -> linear (LDS) -> XYB planes.
-  for (int p = tid; p < T0 * T0; p += kThreads) {
+  // Scale 0: input -> linear (LDS) -> XYB planes. sRGB u8 input: every input
+  // byte of this thread's pixels is loaded before any is used, so a tile waits
+  // out one load latency rather than one per pixel (the XYB stores in between
+  // keep the compiler from hoisting later loads itself).
+  constexpr int NP = T0 * T0 / kThreads;
+  static_assert(NP * kThreads == T0 * T0, "pixels per thread");
+  const bool u8 = in.kind == kInputSrgbU8;
+  unsigned short blo[NI][NP];  // bytes 0, 1 as loaded
+  unsigned char bhi[NI][NP];   // byte 2
+  if (u8) {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int p = tid + k * kThreads;
+      const int gx = min(tx * T0 + p % T0, w - 1), gy = min(ty * T0 + p / T0, h - 1);  // clamped: all lanes load
+#pragma unroll
+      for (int im = 0; im < NI; ++im) {
+        const int img = SIDES == kSidesDis ? 1 : im;
+        gin_u8* px = (gin_u8*)(img ? in.dis : in.ref) + gy * (img ? in.dis_pitch : in.ref_pitch) + gx * 3;
+        __builtin_memcpy(&blo[im][k], (const void*)px, 2);
+        bhi[im][k] = px[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int p = tid + k * kThreads;
     const int lx = p % T0, ly = p / T0;
     const int gx = tx * T0 + lx, gy = ty * T0 + ly;
     if (gx >= w || gy >= h) continue;
+#pragma unroll
     for (int im = 0; im < NI; ++im) {
       const int img = SIDES == kSidesDis ? 1 : im;
       float rgb[3];
-      read_linear(input_of(in, img), gx, gy, lut, rgb);
+      if (u8) {
+        const unsigned lo = blo[im][k];
+        rgb[0] = lut[lo & 255u];
+        rgb[1] = lut[lo >> 8];
+        rgb[2] = lut[bhi[im][k]];
+      } else {
+        read_linear(input_of(in, img), gx, gy, lut, rgb);
+      }
       if (in.kind != kInputLinearF32) {
         float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
         if (lo) {
