@@ -82,6 +82,31 @@ def test_alone_equals_in_the_mix(s2, mix):
     np.testing.assert_array_equal(together[shuffled_idx], shuffled)
 
 
+def test_back_to_back_batches(s2, mix):
+    """Batches enqueued without reading in between (two staging slots): the
+    scores are those of the last batch, a batch on another stream waits for
+    the previous one (shared arena), and every batch scores as it does alone."""
+    _, dev = mix
+    a, b = dev[:5], dev[5:]
+    _say("[gpu] back-to-back batches, two streams")
+    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000) as eng:
+        sa, sb = eng.compute(a), eng.compute(b)
+        st1, st2 = s2.Stream(), s2.Stream()
+        for _ in range(3):
+            eng.compute_async(a, st1)
+            eng.compute_async(b, st1)
+            np.testing.assert_array_equal(eng.get_scores(), sb)
+            eng.compute_async(b, st1)
+            eng.compute_async(a, st2)
+            np.testing.assert_array_equal(eng.get_scores(), sa)
+            for _k in range(4):  # more batches in flight than slots
+                eng.compute_async(a if _k % 2 else b, st2 if _k % 2 else st1)
+            eng.done()
+            np.testing.assert_array_equal(eng.get_scores(), sa)
+        st1.sync()
+        st2.sync()
+
+
 def test_identical_pairs_score_100(s2, synth):
     r, _ = synth.pair(2, 7, 200, 120, 16)
     a = s2.DeviceImage.from_host(r)
