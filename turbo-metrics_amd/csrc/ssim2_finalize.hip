@@ -79,8 +79,20 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const PairDesc* __rest
     }
     lds_sync();
     if (t == 0) {
+      // the weighted sum in cpu.rs's order; nf is a multiple of 18, and each
+      // group of 18 features and weights is loaded before its fma chain (one
+      // load latency per group instead of one per term)
       double ssim = 0.0;
-      for (int k = 0; k < nf; ++k) ssim = fma(c_weight_fin[k], feat[k], ssim);
+      for (int k0 = 0; k0 < nf; k0 += 18) {
+        double fw[18], ww[18];
+#pragma unroll
+        for (int k = 0; k < 18; ++k) {
+          fw[k] = feat[k0 + k];
+          ww[k] = c_weight_fin[k0 + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 18; ++k) ssim = fma(ww[k], fw[k], ssim);
+      }
       ssim *= 0.9562382616834844;
       ssim = fma(6.248496625763138e-5 * ssim * ssim, ssim,
                  fma(2.326765642916932, ssim, -0.020884521182843837 * ssim * ssim));
