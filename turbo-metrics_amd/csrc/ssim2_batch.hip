@@ -138,7 +138,7 @@ void release(ssim2_batch* b) {
 
 ssim2_status enqueue_kernels(ssim2_batch* b, const ssim2_batch::Slot& S, hipStream_t s, int cfg, int mult) {
   const int* meta = meta_ptr(S.d_index);
-  BHIP(launch_fast_pyramid(b->storage, S.d_desc, pyr_prefix(b, S.d_index), meta, b->cus * 4, s));
+  BHIP(launch_fast_pyramid(b->storage, S.d_desc, pyr_prefix(b, S.d_index), meta, b->cus, s));
   BHIP(launch_fast_tile(b->storage, cfg, S.d_desc, tile_prefix(b, S.d_index), meta, b->cus * mult, s));
   BHIP(launch_fast_finalize(S.d_desc, meta, b->cus * 4, (int)b->max_pairs, s));
   return SSIM2_OK;

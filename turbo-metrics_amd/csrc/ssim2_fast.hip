@@ -23,7 +23,8 @@
 //
 // Launches (all persistent: a fixed grid loops over a device-resident item
 // list, so one captured hipGraph serves every batch):
-//   k_fast_pyramid  32x32 scale-0 tiles of every pair: linear pyramid in LDS,
+//   k_fast_pyramid  32x32 scale-0 tiles of every pair: scale-1 pixels straight
+//                   from their 2x2 input blocks, linear scales >= 1 in LDS,
 //                   shifted XYB planes of scales >= 1.
 //   k_fast_tile     one TXxTY output tile of one scale of one pair per item:
 //                   XYB tile + 4 px halo -> products -> horizontal FIR ->
@@ -178,41 +179,51 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
 
 // ---------------------------------------------------------------- pyramid
 
-// sRGB bytes of the pixels one thread converts in a 32x32 tile, for both
+// sRGB bytes of the 2x2 scale-0 block behind one scale-1 pixel, for both
 // images, held as loaded (combined only when converted, so that the loads
 // stay in flight meanwhile).
 struct TileBytes {
-  unsigned short lo[2][kPyrTile * kPyrTile / kThreads];  // bytes 0, 1
-  unsigned char hi[2][kPyrTile * kPyrTile / kThreads];   // byte 2
+  unsigned short lo[2][4];  // bytes 0, 1
+  unsigned char hi[2][4];   // byte 2
 };
 
+// One 32x32 scale-0 tile per item. Scale 0 never goes through LDS: each
+// thread loads the 2x2 input block of its scale-1 pixel (clamped at the image
+// edge, as the downscale clamps) and forms that pixel's mean directly, in the
+// order ((x0,y0) + (x1,y0)) + (x0,y1)) + (x1,y1) of cpu.rs:545-579; only
+// scales >= 1 are kept in LDS (9 KB a block instead of 33 KB, twice the
+// resident blocks).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __restrict__ pairs,
                                                            const int* __restrict__ prefix,
                                                            const int* __restrict__ meta) {
-  constexpr int T0 = kPyrTile, NP = T0 * T0 / kThreads;
-  __shared__ float bufA[6][T0][T0 + 1];
-  __shared__ float bufB[6][T0 / 2][T0 / 2 + 1];
+  constexpr int T0 = kPyrTile, T1 = T0 / 2;
+  static_assert(T1 * T1 == kThreads, "one scale-1 pixel per thread");
+  __shared__ float bufB[6][T1][T1 + 1];          // scales 1, 3, 5
+  __shared__ float bufA[6][T1 / 2][T1 / 2 + 1];  // scales 2, 4
   __shared__ float lut_s[256];  // divergent lookups: LDS, not constant memory
   const int npairs = meta[0], n = prefix[npairs];
   const int tid = threadIdx.x;
+  const int sx = tid % T1, sy = tid / T1;  // this thread's scale-1 pixel in the tile
   for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut_fast[i];
   lds_cfloat* lut = (lds_cfloat*)lut_s;
   ItemRange items(prefix, npairs, n);
+  // Scale-0 coordinates of the 2x2 block, clamped into the image.
+  auto block_xy = [&](const PairDesc& Q, int local, int k, int& gx, int& gy) {
+    const int ntx = (Q.w + T0 - 1) / T0;
+    const int bx = (local % ntx) * T0, by = (local / ntx) * T0;
+    gx = min(bx + 2 * sx + (k & 1), Q.w - 1);
+    gy = min(by + 2 * sy + (k >> 1), Q.h - 1);
+  };
   // Input bytes of sRGB u8 tiles are fetched one item ahead, so their
   // latency overlaps the previous tile's pyramid levels.
   auto fetch = [&](int j, int pj, TileBytes& tb) {
     const PairDesc& Q = pairs[pj];
-    if (Q.in_kind != kInputSrgbU8) return;
-    const int local = j - prefix[pj];
-    const int ntx = (Q.w + T0 - 1) / T0;
-    const int bx = (local % ntx) * T0, by = (local / ntx) * T0;
+    if (Q.in_kind != kInputSrgbU8 || Q.nscales < 2) return;
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      // clamped into the image: every lane loads, the conversion skips
-      // pixels outside
-      const int p = tid + k * kThreads;
-      const int gx = min(bx + p % T0, Q.w - 1), gy = min(by + p / T0, Q.h - 1);
+    for (int k = 0; k < 4; ++k) {
+      int gx, gy;
+      block_xy(Q, j - prefix[pj], k, gx, gy);
 #pragma unroll
       for (int img = 0; img < 2; ++img) {
         gin_u8* px = (gin_u8*)(img ? Q.in_dis : Q.in_ref) + gy * (img ? Q.in_dis_pitch : Q.in_ref_pitch) + gx * 3;
@@ -233,40 +244,52 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     const int local = idx - prefix[pi];
     const int ntx = (P.w + T0 - 1) / T0;
     const int tx = local % ntx, ty = local / ntx;
-    fast_barrier();  // previous item's LDS reads are done
-    if (P.in_kind == kInputSrgbU8) {
+    // scale 1 from the 2x2 input block of this thread's pixel
+    float lin[2][3];
+    if (P.nscales >= 2) {
 #pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        const int p = tid + k * kThreads;
-        const int lx = p % T0, ly = p / T0;
-        if (tx * T0 + lx >= P.w || ty * T0 + ly >= P.h) continue;
+      for (int img = 0; img < 2; ++img) {
+        float px4[4][3];
 #pragma unroll
-        for (int img = 0; img < 2; ++img) {
-          const unsigned lo = tb.lo[img][k];
-          bufA[img * 3 + 0][ly][lx] = lut[lo & 255u];
-          bufA[img * 3 + 1][ly][lx] = lut[lo >> 8];
-          bufA[img * 3 + 2][ly][lx] = lut[tb.hi[img][k]];
+        for (int k = 0; k < 4; ++k) {
+          if (P.in_kind == kInputSrgbU8) {
+            const unsigned lo = tb.lo[img][k];
+            px4[k][0] = lut[lo & 255u];
+            px4[k][1] = lut[lo >> 8];
+            px4[k][2] = lut[tb.hi[img][k]];
+          } else {
+            int gx, gy;
+            block_xy(P, local, k, gx, gy);
+            read_linear(input_of(P, img), gx, gy, lut, px4[k]);
+          }
         }
-      }
-    } else {
-      for (int p = tid; p < T0 * T0; p += kThreads) {
-        const int lx = p % T0, ly = p / T0;
-        const int gx = tx * T0 + lx, gy = ty * T0 + ly;
-        if (gx >= P.w || gy >= P.h) continue;
 #pragma unroll
-        for (int img = 0; img < 2; ++img) {
-          float rgb[3];
-          read_linear(input_of(P, img), gx, gy, lut, rgb);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) bufA[img * 3 + c][ly][lx] = rgb[c];
-        }
+        for (int c = 0; c < 3; ++c) lin[img][c] = (((px4[0][c] + px4[1][c]) + px4[2][c]) + px4[3][c]) * 0.25f;
       }
     }
     if (idx + 1 < items.end) {
       pi_next = items.pair_of(idx + 1);
       fetch(idx + 1, pi_next, tb);
     }
-    for (int s = 1; s < P.nscales; ++s) {
+    if (P.nscales < 2) continue;  // uniform: the whole block has this item
+    fast_barrier();  // the previous item's LDS reads are done
+    {
+      const int gx = tx * T1 + sx, gy = ty * T1 + sy;
+#pragma unroll
+      for (int img = 0; img < 2; ++img) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bufB[img * 3 + c][sy][sx] = lin[img][c];
+        if (gx < P.sc[1].w && gy < P.sc[1].h) {
+          float X, Y, B;
+          xyb_fast(lin[img][0], lin[img][1], lin[img][2], X, Y, B);
+          const long long off = (long long)gy * P.sc[1].pitch + gx;
+          fast_plane<T>(P, 1, img, 0)[off] = Codec<T>::enc(X - xyb_shift(0));
+          fast_plane<T>(P, 1, img, 1)[off] = Codec<T>::enc(Y - xyb_shift(1));
+          fast_plane<T>(P, 1, img, 2)[off] = Codec<T>::enc(B - xyb_shift(2));
+        }
+      }
+    }
+    for (int s = 2; s < P.nscales; ++s) {
       fast_barrier();
       const int Ts = T0 >> s;
       const int pw = P.sc[s - 1].w, ph = P.sc[s - 1].h;
@@ -505,17 +528,38 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
 
 // ---------------------------------------------------------------- launchers
 
+// Persistent grid: as many blocks as the CUs hold at once (queried once per
+// storage type), so every block's static item range starts at launch.
+template <typename T>
+static int pyramid_blocks_per_cu() {
+  static int n = 0;
+  if (n == 0) {
+    int v = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, reinterpret_cast<const void*>(&k_fast_pyramid<T>), kThreads,
+                                                     0) != hipSuccess ||
+        v < 1) {
+      (void)hipGetLastError();
+      v = 4;
+    }
+    n = v;
+  }
+  return n;
+}
+
 hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta,
-                               int grid, hipStream_t stream) {
+                               int cus, hipStream_t stream) {
   switch (storage) {
     case kStorageF32:
-      hipLaunchKernelGGL(k_fast_pyramid<float>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      hipLaunchKernelGGL(k_fast_pyramid<float>, dim3(cus * pyramid_blocks_per_cu<float>()), dim3(kThreads), 0, stream,
+                         pairs, prefix, meta);
       break;
     case kStorageF16:
-      hipLaunchKernelGGL(k_fast_pyramid<_Float16>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      hipLaunchKernelGGL(k_fast_pyramid<_Float16>, dim3(cus * pyramid_blocks_per_cu<_Float16>()), dim3(kThreads), 0,
+                         stream, pairs, prefix, meta);
       break;
     default:
-      hipLaunchKernelGGL(k_fast_pyramid<bf16_t>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      hipLaunchKernelGGL(k_fast_pyramid<bf16_t>, dim3(cus * pyramid_blocks_per_cu<bf16_t>()), dim3(kThreads), 0,
+                         stream, pairs, prefix, meta);
       break;
   }
   return hipGetLastError();

@@ -136,7 +136,7 @@ hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream
 // Fast form (ssim2_fast.hip). `prefix` holds npairs+1 first-item indices,
 // `meta[0]` the pair count; grids are fixed (persistent kernels).
 hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta,
-                               int grid, hipStream_t stream);
+                               int cus, hipStream_t stream);  // grid: cus x resident blocks per CU
 hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, const int* prefix,
                             const int* meta, int grid, hipStream_t stream);
 hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, int max_pairs,
