@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
                                                       InputArgs in) {
   constexpr int NI = SideTraits<SIDES>::NI;
   constexpr int T0 = kPyrTile;
-  __shared__ float bufA[NI * 3][T0][T0 + 1];          // scales 0, 2, 4
+  __shared__ float bufA[NI * 3][T0 / 4][T0 / 4 + 1];  // scales 2, 4
   __shared__ float bufB[NI * 3][T0 / 2][T0 / 2 + 1];  // scales 1, 3, 5
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
@@ -144,20 +144,21 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
   }
   lds_cfloat* lut = (lds_cfloat*)lut_s;
 
-  // Scale 0: input -> linear (LDS) -> XYB planes. sRGB u8 input: every input
-  // byte of this thread's pixels is loaded before any is used, so a tile waits
-  // out one load latency rather than one per pixel (the XYB stores in between
-  // keep the compiler from hoisting later loads itself).
-  constexpr int NP = T0 * T0 / kThreads;
-  static_assert(NP * kThreads == T0 * T0, "pixels per thread");
+  // Scales 0 and 1 without a scale-0 tile: each thread takes the 2x2 scale-0
+  // block behind its scale-1 pixel (clamped at the image edge, as the
+  // downscale clamps), converts and stores the block's in-image pixels, and
+  // forms the scale-1 mean from the same registers. sRGB u8 input: all input
+  // bytes of the block are loaded before the first is used.
+  constexpr int T1 = T0 / 2;
+  static_assert(T1 * T1 == kThreads, "one scale-1 pixel per thread");
+  const int sx = tid % T1, sy = tid / T1;
   const bool u8 = in.kind == kInputSrgbU8;
-  unsigned short blo[NI][NP];  // bytes 0, 1 as loaded
-  unsigned char bhi[NI][NP];   // byte 2
+  unsigned short blo[NI][4];  // bytes 0, 1 as loaded
+  unsigned char bhi[NI][4];   // byte 2
   if (u8) {
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      const int p = tid + k * kThreads;
-      const int gx = min(tx * T0 + p % T0, w - 1), gy = min(ty * T0 + p / T0, h - 1);  // clamped: all lanes load
+    for (int k = 0; k < 4; ++k) {
+      const int gx = min(tx * T0 + 2 * sx + (k & 1), w - 1), gy = min(ty * T0 + 2 * sy + (k >> 1), h - 1);
 #pragma unroll
       for (int im = 0; im < NI; ++im) {
         const int img = SIDES == kSidesDis ? 1 : im;
@@ -167,12 +168,18 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
       }
     }
   }
+  float sum1[NI][3];
 #pragma unroll
-  for (int k = 0; k < NP; ++k) {
-    const int p = tid + k * kThreads;
-    const int lx = p % T0, ly = p / T0;
-    const int gx = tx * T0 + lx, gy = ty * T0 + ly;
-    if (gx >= w || gy >= h) continue;
+  for (int im = 0; im < NI; ++im)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sum1[im][c] = 0.0f;
+  // block pixel k = (x0 + (k & 1), y0 + (k >> 1)): the summation order of
+  // cpu.rs:545-579 is (x0,y0), (x1,y0), (x0,y1), (x1,y1)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int ux = tx * T0 + 2 * sx + (k & 1), uy = ty * T0 + 2 * sy + (k >> 1);  // unclamped
+    const int gx = min(ux, w - 1), gy = min(uy, h - 1);
+    const bool inside = ux < w && uy < h;
 #pragma unroll
     for (int im = 0; im < NI; ++im) {
       const int img = SIDES == kSidesDis ? 1 : im;
@@ -185,6 +192,9 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
       } else {
         read_linear(input_of(in, img), gx, gy, lut, rgb);
       }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sum1[im][c] += rgb[c];
+      if (!inside) continue;
       if (in.kind != kInputLinearF32) {
         float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
         if (lo) {
@@ -201,12 +211,32 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
       gbl(xyb_plane(P, 0, img, 0, SIDES))[off] = X;
       gbl(xyb_plane(P, 0, img, 1, SIDES))[off] = Y;
       gbl(xyb_plane(P, 0, img, 2, SIDES))[off] = B;
-      for (int c = 0; c < 3; ++c) bufA[im * 3 + c][ly][lx] = rgb[c];
+    }
+  }
+  if (P.nscales > 1) {
+    const int gx = tx * T1 + sx, gy = ty * T1 + sy;
+#pragma unroll
+    for (int im = 0; im < NI; ++im) {
+      const int img = SIDES == kSidesDis ? 1 : im;
+      float rgb[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        rgb[c] = sum1[im][c] * 0.25f;
+        bufB[im * 3 + c][sy][sx] = rgb[c];
+      }
+      if (gx < P.sc[1].w && gy < P.sc[1].h) {
+        float X, Y, B;
+        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
+        const long long off = (long long)gy * P.sc[1].pitch + gx;
+        gbl(xyb_plane(P, 1, img, 0, SIDES))[off] = X;
+        gbl(xyb_plane(P, 1, img, 1, SIDES))[off] = Y;
+        gbl(xyb_plane(P, 1, img, 2, SIDES))[off] = B;
+      }
     }
   }
 
-  // Scales 1..n-1: 2x2 mean of the previous scale (cpu.rs:545-579), then XYB.
-  for (int s = 1; s < P.nscales; ++s) {
+  // Scales 2..n-1: 2x2 mean of the previous scale (cpu.rs:545-579), then XYB.
+  for (int s = 2; s < P.nscales; ++s) {
     lds_barrier();
     const int T = T0 >> s;
     const int pw = P.sc[s - 1].w, ph = P.sc[s - 1].h;
@@ -259,8 +289,8 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 // One wave per (scale, channel, quantity, band of 20 rows). The scan along a
 // row is sequential (cpu.rs:967-1022, row form), so its time is the number of
 // instructions a wave issues per column: each row is spread over three lanes,
-// one per cosine term, which issue one recurrence each per column and combine
-// the terms with two DPP adds. That leaves more waves than SIMDs at 1080p.
+// one per cosine term, which issue one recurrence each per column; the terms
+// are summed in the store phase. That leaves more waves than SIMDs at 1080p.
 //
 // Memory is staged through LDS in chunks of 20 rows x 32 columns so that every
 // global access is coalesced (8 rows x 128 B per instruction, instead of one
