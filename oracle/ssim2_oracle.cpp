@@ -34,6 +34,9 @@
 //                          mu  = blur(x') + c*b1,   x - mu = (x' - blur(x')) + c*e
 //   MODE_FAST_SHIFT_F16 (8) MODE_FAST_SHIFT with the XYB planes of scales >= 1
 //                        stored in binary16 as in MODE_FAST_F16.
+//   MODE_FAST_SHIFT_F16_ALL (9) MODE_FAST_SHIFT with the XYB planes of every
+//                        scale, scale 0 included, stored in binary16 (the
+//                        FP16-storage form keeps all scales in HBM).
 //
 // Parity pins (each checked in tests/):
 //   * LUT and WEIGHT tables: equal, bit for bit, to the tables extracted from
@@ -455,7 +458,7 @@ int pipeline(const T* ref_lin, const T* dis_lin, int w0, int h0, int mode, doubl
           }
           xyb_fast(l1[0], l1[1], l1[2], &a[0], &a[1], &a[2]);
           xyb_fast(l2[0], l2[1], l2[2], &b[0], &b[1], &b[2]);
-          if (((mode == 4 || mode == 8) && scale > 0) || mode == 5) {
+          if (((mode == 4 || mode == 8) && scale > 0) || mode == 5 || mode == 9) {
             for (int c = 0; c < 3; ++c) {
               a[c] = round_f16(a[c] - kXybShift[c]) + kXybShift[c];
               b[c] = round_f16(b[c] - kXybShift[c]) + kXybShift[c];
@@ -487,7 +490,7 @@ int pipeline(const T* ref_lin, const T* dis_lin, int w0, int h0, int mode, doubl
     };
     double* ssc = sums + size_t(scale) * 18;
     if constexpr (sizeof(T) == 4) {
-      if (mode == 7 || mode == 8) {
+      if (mode == 7 || mode == 8 || mode == 9) {
         shifted_scale_sums(p1, p2, w, h, ssc);
         dims[scale * 2] = w;
         dims[scale * 2 + 1] = h;

@@ -39,7 +39,7 @@ struct FastPlan {
   PairDesc d{};
   int npyr = 0;
   int ntiles = 0;
-  size_t arena = 0;  // bytes of XYB planes (scales >= 1)
+  size_t arena = 0;  // bytes of XYB planes (all scales)
   int nparts = 0;
 };
 
@@ -62,10 +62,8 @@ bool plan_fast(int w, int h, int storage, int cfg, size_t arena_off, int part_ba
     S.h = sh[s];
     S.pitch = (sw[s] + 15) / 16 * 16;
     S.plane = (long long)S.pitch * S.h;
-    if (s > 0) {
-      S.xyb = (long long)off;
-      off += align256(6 * (size_t)S.plane * sb);
-    }
+    S.xyb = (long long)off;  // every scale, scale 0 included
+    off += align256(6 * (size_t)S.plane * sb);
     S.tiles_x = (sw[s] + TX - 1) / TX;
     S.colblocks = S.tiles_x * ((sh[s] + TY - 1) / TY);
     S.tile0 = tile;
@@ -310,10 +308,11 @@ ssim2_status ssim2_batch_create(const ssim2_batch_options* o, ssim2_batch** out)
   b->max_pairs = o->max_pairs;
   b->max_pixels = o->max_pixels;
   const size_t sb = storage_bytes(b->storage);
-  // Scales >= 1 hold at most ~1/3 of the scale-0 pixels; pitch rounding adds
-  // at most 15 columns per plane row. A batch whose exact need exceeds this
-  // capacity is refused with SSIM2_BATCH_SIZE (split it).
-  b->arena_cap = 6 * sb * (size_t(o->max_pixels) / 2 + size_t(o->max_pairs) * 32768) +
+  // All scales hold at most ~4/3 of the scale-0 pixels (plus the odd-size
+  // rounding); pitch rounding adds at most 15 columns per plane row. A batch
+  // whose exact need exceeds this capacity is refused with SSIM2_BATCH_SIZE
+  // (split it).
+  b->arena_cap = 6 * sb * (size_t(o->max_pixels) * 3 / 2 + size_t(o->max_pairs) * 65536) +
                  size_t(256) * kMaxScales * o->max_pairs;
   b->part_cap = size_t(o->max_pixels) * 3 / 128 + size_t(o->max_pairs) * 3 * 2048;
   const size_t index_ints = 4 + 2 * size_t(o->max_pairs + 1);

@@ -189,10 +189,11 @@ struct TileBytes {
 
 // One 32x32 scale-0 tile per item. Scale 0 never goes through LDS: each
 // thread loads the 2x2 input block of its scale-1 pixel (clamped at the image
-// edge, as the downscale clamps) and forms that pixel's mean directly, in the
-// order ((x0,y0) + (x1,y0)) + (x0,y1)) + (x1,y1) of cpu.rs:545-579; only
-// scales >= 1 are kept in LDS (9 KB a block instead of 33 KB, twice the
-// resident blocks).
+// edge, as the downscale clamps), stores the shifted XYB of the block's
+// in-image pixels (scale 0 of every plane is computed here once, not per tile
+// halo in the tile kernel) and forms the scale-1 mean directly, in the order
+// ((x0,y0) + (x1,y0)) + (x0,y1)) + (x1,y1) of cpu.rs:545-579; only scales
+// >= 1 are kept in LDS (9 KB a block).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __restrict__ pairs,
                                                            const int* __restrict__ prefix,
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
   // latency overlaps the previous tile's pyramid levels.
   auto fetch = [&](int j, int pj, TileBytes& tb) {
     const PairDesc& Q = pairs[pj];
-    if (Q.in_kind != kInputSrgbU8 || Q.nscales < 2) return;
+    if (Q.in_kind != kInputSrgbU8) return;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       int gx, gy;
@@ -244,28 +245,38 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     const int local = idx - prefix[pi];
     const int ntx = (P.w + T0 - 1) / T0;
     const int tx = local % ntx, ty = local / ntx;
-    // scale 1 from the 2x2 input block of this thread's pixel
+    // scale 0 (stored) and the scale-1 mean from the 2x2 input block of this
+    // thread's scale-1 pixel; the fetched bytes are the last loads issued,
+    // and stores issued since may complete before them
+    if (P.in_kind == kInputSrgbU8) __builtin_amdgcn_s_waitcnt(vmcnt_only<0>());
     float lin[2][3];
-    if (P.nscales >= 2) {
 #pragma unroll
-      for (int img = 0; img < 2; ++img) {
-        float px4[4][3];
+    for (int img = 0; img < 2; ++img) {
+      float px4[4][3];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if (P.in_kind == kInputSrgbU8) {
-            const unsigned lo = tb.lo[img][k];
-            px4[k][0] = lut[lo & 255u];
-            px4[k][1] = lut[lo >> 8];
-            px4[k][2] = lut[tb.hi[img][k]];
-          } else {
-            int gx, gy;
-            block_xy(P, local, k, gx, gy);
-            read_linear(input_of(P, img), gx, gy, lut, px4[k]);
-          }
+      for (int k = 0; k < 4; ++k) {
+        int gx, gy;
+        block_xy(P, local, k, gx, gy);
+        if (P.in_kind == kInputSrgbU8) {
+          const unsigned lo = tb.lo[img][k];
+          px4[k][0] = lut[lo & 255u];
+          px4[k][1] = lut[lo >> 8];
+          px4[k][2] = lut[tb.hi[img][k]];
+        } else {
+          read_linear(input_of(P, img), gx, gy, lut, px4[k]);
         }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) lin[img][c] = (((px4[0][c] + px4[1][c]) + px4[2][c]) + px4[3][c]) * 0.25f;
+        const int ux = tx * T0 + 2 * sx + (k & 1), uy = ty * T0 + 2 * sy + (k >> 1);  // unclamped
+        if (ux < P.w && uy < P.h) {
+          float X, Y, B;
+          xyb_fast(px4[k][0], px4[k][1], px4[k][2], X, Y, B);
+          const long long off = (long long)uy * P.sc[0].pitch + ux;
+          fast_plane<T>(P, 0, img, 0)[off] = Codec<T>::enc(X - xyb_shift(0));
+          fast_plane<T>(P, 0, img, 1)[off] = Codec<T>::enc(Y - xyb_shift(1));
+          fast_plane<T>(P, 0, img, 2)[off] = Codec<T>::enc(B - xyb_shift(2));
+        }
       }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lin[img][c] = (((px4[0][c] + px4[1][c]) + px4[2][c]) + px4[3][c]) * 0.25f;
     }
     if (idx + 1 < items.end) {
       pi_next = items.pair_of(idx + 1);
@@ -361,30 +372,18 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
     const bool interior = x0 >= 4 && x0 + TX <= w - 4 && y0 >= 4 && y0 + TY <= h - 4;
     fast_barrier();  // previous item's LDS reads are done
 
-    // Phase 1: shifted XYB of the tile and its 4 px halo.
+    // Phase 1: shifted XYB of the tile and its 4 px halo, from the planes the
+    // pyramid wrote (every scale, scale 0 included).
     for (int e = tid; e < IH * IW; e += kThreads) {
       const int iy = e / IW, ix = e % IW;
       const int gx = x0 - 4 + ix, gy = y0 - 4 + iy;
       float v[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
       if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
-        if (s == 0) {
+        const long long off = (long long)gy * S.pitch + gx;
 #pragma unroll
-          for (int img = 0; img < 2; ++img) {
-            float rgb[3];
-            read_linear(input_of(P, img), gx, gy, c_lut_fast, rgb);
-            float X, Y, B;
-            xyb_fast(rgb[0], rgb[1], rgb[2], X, Y, B);
-            v[img][0] = X - xyb_shift(0);
-            v[img][1] = Y - xyb_shift(1);
-            v[img][2] = B - xyb_shift(2);
-          }
-        } else {
-          const long long off = (long long)gy * S.pitch + gx;
+        for (int img = 0; img < 2; ++img)
 #pragma unroll
-          for (int img = 0; img < 2; ++img)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[img][c] = Codec<T>::dec(fast_plane<T>(P, s, img, c)[off]);
-        }
+          for (int c = 0; c < 3; ++c) v[img][c] = Codec<T>::dec(fast_plane<T>(P, s, img, c)[off]);
       }
 #pragma unroll
       for (int img = 0; img < 2; ++img)

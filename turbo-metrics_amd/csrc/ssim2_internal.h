@@ -117,6 +117,17 @@ struct InputView {
 };
 
 // One work item per block: {pair, a, b, c}; the meaning depends on the kernel.
+// s_waitcnt immediate that waits for vmcnt <= N only (gfx9 encoding: vmcnt
+// bits 3:0 and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima).
+// Loads complete in issue order, stores may complete before earlier loads:
+// to know a load done, wait for N = the loads issued after it, whatever
+// stores came in between.
+template <int N>
+constexpr int vmcnt_only() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  return (N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14);
+}
+
 struct WorkItem {
   int pair, a, b, c;
 };

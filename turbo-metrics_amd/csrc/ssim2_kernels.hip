@@ -322,14 +322,6 @@ __device__ __forceinline__ float4 u2f(u32x4 v) {
 // reordering has to be stopped between the phases.
 __device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory"); }
 
-// s_waitcnt immediate that waits for vmcnt <= N only (gfx9 encoding: vmcnt
-// bits 3:0 and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima).
-template <int N>
-constexpr int vmcnt_only() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  return (N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14);
-}
-
 template <int Q>
 __device__ __forceinline__ float4 row_quantity(float4 x, float4 y) {
   if (Q == 0) return make_float4(x.x * x.x, x.y * x.y, x.z * x.z, x.w * x.w);
