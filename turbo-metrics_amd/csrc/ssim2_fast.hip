@@ -111,6 +111,18 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* fast_plane(const
                                                (long long)(img * 3 + c) * S.plane);
 }
 typedef __attribute__((address_space(1))) const unsigned char gcu8;
+template <typename T>
+using gT = __attribute__((address_space(1))) T;
+
+// Two adjacent elements as one store; q must be aligned to two elements.
+template <typename T>
+__device__ __forceinline__ void store_pair(gT<T>* q, T a, T b) {
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  V2 v;
+  v.x = a;
+  v.y = b;
+  *(gT<V2>*)q = v;
+}
 
 // Sum of the in-range taps of the 1-D FIR at position g of a line of n (the
 // 1-D blur of ones); exactly 1 away from the border.
@@ -245,43 +257,71 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     const int local = idx - prefix[pi];
     const int ntx = (P.w + T0 - 1) / T0;
     const int tx = local % ntx, ty = local / ntx;
-    // scale 0 (stored) and the scale-1 mean from the 2x2 input block of this
-    // thread's scale-1 pixel; the fetched bytes are the last loads issued,
-    // and stores issued since may complete before them
-    if (P.in_kind == kInputSrgbU8) __builtin_amdgcn_s_waitcnt(vmcnt_only<0>());
-    float lin[2][3];
+    // Linear RGB of this thread's 2x2 block in both images (the branch on
+    // the input kind is uniform over the block).
+    float px[2][4][3];
+    if (P.in_kind == kInputSrgbU8) {
+      // the fetched bytes are the last loads issued; stores issued since
+      // may complete before them, so only a full wait proves them done
+      __builtin_amdgcn_s_waitcnt(vmcnt_only<0>());
 #pragma unroll
-    for (int img = 0; img < 2; ++img) {
-      float px4[4][3];
+      for (int img = 0; img < 2; ++img)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int gx, gy;
-        block_xy(P, local, k, gx, gy);
-        if (P.in_kind == kInputSrgbU8) {
+        for (int k = 0; k < 4; ++k) {
           const unsigned lo = tb.lo[img][k];
-          px4[k][0] = lut[lo & 255u];
-          px4[k][1] = lut[lo >> 8];
-          px4[k][2] = lut[tb.hi[img][k]];
-        } else {
-          read_linear(input_of(P, img), gx, gy, lut, px4[k]);
+          px[img][k][0] = lut[lo & 255u];
+          px[img][k][1] = lut[lo >> 8];
+          px[img][k][2] = lut[tb.hi[img][k]];
         }
-        const int ux = tx * T0 + 2 * sx + (k & 1), uy = ty * T0 + 2 * sy + (k >> 1);  // unclamped
-        if (ux < P.w && uy < P.h) {
-          float X, Y, B;
-          xyb_fast(px4[k][0], px4[k][1], px4[k][2], X, Y, B);
-          const long long off = (long long)uy * P.sc[0].pitch + ux;
-          fast_plane<T>(P, 0, img, 0)[off] = Codec<T>::enc(X - xyb_shift(0));
-          fast_plane<T>(P, 0, img, 1)[off] = Codec<T>::enc(Y - xyb_shift(1));
-          fast_plane<T>(P, 0, img, 2)[off] = Codec<T>::enc(B - xyb_shift(2));
-        }
-      }
+    } else {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) lin[img][c] = (((px4[0][c] + px4[1][c]) + px4[2][c]) + px4[3][c]) * 0.25f;
+      for (int img = 0; img < 2; ++img)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          int gx, gy;
+          block_xy(P, local, k, gx, gy);
+          read_linear(input_of(P, img), gx, gy, lut, px[img][k]);
+        }
     }
     if (idx + 1 < items.end) {
       pi_next = items.pair_of(idx + 1);
       fetch(idx + 1, pi_next, tb);
     }
+    // Scale 0: shifted XYB of the block's in-image rows, each row of the
+    // block as one two-element store. ux is even and so is the pitch, so the
+    // pair is aligned and stays inside its plane row; at an odd width its
+    // second element lands in the row's padding column, which no reader
+    // takes as a pixel.
+    const int ux = tx * T0 + 2 * sx, uy = ty * T0 + 2 * sy;
+    const bool in0 = ux < P.w;
+#pragma unroll
+    for (int img = 0; img < 2; ++img) {
+      float v[4][3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float X, Y, B;
+        xyb_fast(px[img][k][0], px[img][k][1], px[img][k][2], X, Y, B);
+        v[k][0] = X - xyb_shift(0);
+        v[k][1] = Y - xyb_shift(1);
+        v[k][2] = B - xyb_shift(2);
+      }
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        if (!in0 || uy + r >= P.h) continue;
+        const long long off = (long long)(uy + r) * P.sc[0].pitch + ux;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          store_pair(fast_plane<T>(P, 0, img, c) + off, Codec<T>::enc(v[2 * r][c]), Codec<T>::enc(v[2 * r + 1][c]));
+        }
+      }
+    }
+    // scale-1 mean in the order of cpu.rs:545-579
+    float lin[2][3];
+#pragma unroll
+    for (int img = 0; img < 2; ++img)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        lin[img][c] = (((px[img][0][c] + px[img][1][c]) + px[img][2][c]) + px[img][3][c]) * 0.25f;
     if (P.nscales < 2) continue;  // uniform: the whole block has this item
     fast_barrier();  // the previous item's LDS reads are done
     {
@@ -344,18 +384,38 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
 // ---------------------------------------------------------------- fused tile
 
 // RU = output rows per vertical-pass unit (TY / RU units per column).
+// One channel at a time is in LDS (a quarter of the block's LDS for all
+// three); the next channel's halo is loaded into registers, four elements
+// a load, while the current one is filtered.
+#ifndef SSIM2_TILE_WAVES_32X16
+#define SSIM2_TILE_WAVES_32X16 6
+#endif
+// Waves per SIMD the registers of a tile shape are sized for (1: no bound).
+template <int TX, int TY>
+constexpr int tile_waves() {
+  return TX == 32 && TY == 16 ? SSIM2_TILE_WAVES_32X16 : 1;
+}
+
 template <typename T, int TX, int TY, int RU>
-__global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_waves<TX, TY>())))
+void k_fast_tile(const PairDesc* __restrict__ pairs,
                                                         const int* __restrict__ prefix,
                                                         const int* __restrict__ meta) {
   static_assert(TX % 4 == 0 && TY % RU == 0, "tile shape");
   constexpr int IW = TX + 8, IH = TY + 8;
+  constexpr int XS = IW + 4, HS = TX + 4;  // LDS row strides (16-byte rows)
+  constexpr int CW = IW / 4, NCH = IH * CW;  // 4-element chunks of one halo plane
+  constexpr int NCT = (NCH + kThreads - 1) / kThreads;
   // Blurred quantities: r'^2 + d'^2, r'd', r', d'. The maps use the two
   // variances only through their sum, so one blur serves both.
   constexpr int kQ = 4;
-  __shared__ float xs[2][3][IH][IW + 1];  // shifted XYB, zero outside the image
-  __shared__ float hq[kQ][IH][TX + 1];    // horizontal pass, one channel at a time
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  typedef float F4 __attribute__((ext_vector_type(4)));
+  __shared__ F4 xs4[2][IH][XS / 4];  // shifted XYB of one channel, zero outside the image
+  __shared__ F4 hq4[kQ][IH][HS / 4];  // horizontal pass of that channel
   __shared__ double red[kThreads / 64][18];
+  float (*xs)[IH][XS] = reinterpret_cast<float (*)[IH][XS]>(&xs4[0][0][0]);
+  float (*hq)[IH][HS] = reinterpret_cast<float (*)[IH][HS]>(&hq4[0][0][0]);
   const int npairs = meta[0], n = prefix[npairs];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   ItemRange items(prefix, npairs, n);
@@ -370,27 +430,45 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
     const int tx = tile % S.tiles_x, ty = tile / S.tiles_x;
     const int x0 = tx * TX, y0 = ty * TY;
     const bool interior = x0 >= 4 && x0 + TX <= w - 4 && y0 >= 4 && y0 + TY <= h - 4;
-    fast_barrier();  // previous item's LDS reads are done
 
-    // Phase 1: shifted XYB of the tile and its 4 px halo, from the planes the
-    // pyramid wrote (every scale, scale 0 included).
-    for (int e = tid; e < IH * IW; e += kThreads) {
-      const int iy = e / IW, ix = e % IW;
-      const int gx = x0 - 4 + ix, gy = y0 - 4 + iy;
-      float v[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-      if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
-        const long long off = (long long)gy * S.pitch + gx;
+    // Halo chunks of channel c into registers. A chunk starts at a multiple
+    // of 4 (x0 is one) and so does every plane row (the pitch is a multiple
+    // of 16), so a chunk is one aligned 4-element load; a chunk that starts
+    // inside the image ends inside its row, the columns past w being the
+    // row's padding, zeroed when the chunk is written to LDS.
+    V4 raw[NCT][2];
+    auto load = [&](int c) {
 #pragma unroll
-        for (int img = 0; img < 2; ++img)
+      for (int t = 0; t < NCT; ++t) {
+        const int e = tid + t * kThreads;
+        const int iy = e / CW, j = e % CW;
+        const int gx = x0 - 4 + 4 * j, gy = y0 - 4 + iy;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) v[img][c] = Codec<T>::dec(fast_plane<T>(P, s, img, c)[off]);
+        for (int img = 0; img < 2; ++img) {
+          if (e < NCH && gx >= 0 && gx < w && gy >= 0 && gy < h)
+            raw[t][img] = *(const gT<V4>*)(fast_plane<T>(P, s, img, c) + (long long)gy * S.pitch + gx);
+          else
+            raw[t][img] = V4{};
+        }
       }
+    };
+    auto to_lds = [&]() {
 #pragma unroll
-      for (int img = 0; img < 2; ++img)
+      for (int t = 0; t < NCT; ++t) {
+        const int e = tid + t * kThreads;
+        if (e >= NCH) continue;
+        const int iy = e / CW, j = e % CW;
+        const int gx = x0 - 4 + 4 * j;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xs[img][c][iy][ix] = v[img][c];
-    }
-
+        for (int img = 0; img < 2; ++img) {
+          F4 f;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) f[k] = gx + k < w ? Codec<T>::dec(raw[t][img][k]) : 0.0f;
+          xs4[img][iy][j] = f;
+        }
+      }
+    };
+    load(0);
     float acc[3][6];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -398,15 +476,22 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
       for (int j = 0; j < 6; ++j) acc[c][j] = 0.0f;
 
     for (int c = 0; c < 3; ++c) {
-      fast_barrier();  // xs complete (c = 0) / previous channel's hq reads done
-      // Phase 2: horizontal FIR of the four quantities, 4 outputs per unit.
+      fast_barrier();  // the previous channel's (or item's) LDS reads are done
+      to_lds();
+      if (c < 2) load(c + 1);
+      fast_barrier();
+      // Horizontal FIR of the four quantities, 4 outputs per unit.
       for (int e = tid; e < IH * (TX / 4); e += kThreads) {
-        const int iy = e / (TX / 4), ox0 = (e % (TX / 4)) * 4;
+        const int iy = e / (TX / 4), j0 = e % (TX / 4), ox0 = j0 * 4;
         float a[12], d[12];
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
-          a[k] = xs[0][c][iy][ox0 + k];
-          d[k] = xs[1][c][iy][ox0 + k];
+        for (int m = 0; m < 3; ++m) {
+          const F4 va = xs4[0][iy][j0 + m], vd = xs4[1][iy][j0 + m];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            a[4 * m + k] = va[k];
+            d[4 * m + k] = vd[k];
+          }
         }
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
@@ -414,6 +499,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
 #pragma unroll
           for (int k = 0; k < 12; ++k)
             qv[k] = q == 0 ? a[k] * a[k] + d[k] * d[k] : (q == 1 ? a[k] * d[k] : (q == 2 ? a[k] : d[k]));
+          F4 r4;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             float r = kH4 * (qv[j] + qv[j + 8]);
@@ -421,12 +507,13 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
             r = fmaf(kH2, qv[j + 2] + qv[j + 6], r);
             r = fmaf(kH1, qv[j + 3] + qv[j + 5], r);
             r = fmaf(kH0, qv[j + 4], r);
-            hq[q][iy][ox0 + j] = r;
+            r4[j] = r;
           }
+          hq4[q][iy][j0] = r4;
         }
       }
       fast_barrier();
-      // Phase 3: vertical FIR (RU rows per unit), statistics, error maps.
+      // Vertical FIR (RU rows per unit), statistics, error maps.
       // A tile whose outputs all lie 4 px or more inside the image has
       // b1 = 1 and e = 0 everywhere, so its statistics need none of the
       // zero-padding terms (the choice is uniform over the block).
@@ -484,7 +571,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_tile(const PairDesc* __restri
             const float d2 = dd * dd;
             acc[c][0] += dd;
             acc[c][1] += d2 * d2;
-            const float src = xs[0][c][oy0 + j + 4][ox + 4], dis = xs[1][c][oy0 + j + 4][ox + 4];
+            const float src = xs[0][oy0 + j + 4][ox + 4], dis = xs[1][oy0 + j + 4][ox + 4];
             float rn = dis - m2, rd = src - m1;
             if constexpr (kEdge) {
               rn += ec;
