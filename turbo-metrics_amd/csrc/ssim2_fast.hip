@@ -54,22 +54,25 @@ __device__ __forceinline__ float xyb_shift(int c) { return c == 0 ? 0.5f : (c ==
 // Cube root as exp2(log2(x) / 3) on the hardware log2/exp2 (about 1 ulp
 // each): a few ulp in all, far below the binary16 storage quantum and the
 // FIR's own rounding, and two quarter-rate operations instead of three.
+// x <= 0 gives 0 (log2(0) = -inf, exp2(-inf) = 0).
 __device__ __forceinline__ float cbrt_fast(float x) {
-  const float y = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * 0.333333343f);
-  return x > 0.0f ? y : 0.0f;
+  return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(fmaxf(x, 0.0f)) * 0.333333343f);
 }
 
-__device__ __forceinline__ void xyb_fast(float r, float g, float b, float& X, float& Y, float& B) {
-  float rg = fmaf(kM00, r, fmaf(kM01, g, fmaf(kM02, b, kB0)));
-  float gr = fmaf(kM10, r, fmaf(kM11, g, fmaf(kM12, b, kB0)));
-  float bb = fmaf(kM20, r, fmaf(kM21, g, fmaf(kM22, b, kB0)));
-  rg = cbrt_fast(rg) - kB0Root;
-  gr = cbrt_fast(gr) - kB0Root;
-  bb = cbrt_fast(bb) - kB0Root;
-  const float x = 0.5f * (rg - gr), y = 0.5f * (rg + gr);
-  X = fmaf(x, 14.0f, 0.42f);
-  Y = y + 0.01f;
-  B = (bb - y) + 0.55f;
+// Shifted XYB (value minus xyb_shift) of linear RGB: cpu.rs:458-496 and
+// make_positive_xyb (cpu.rs:509-515) with the constant terms regrouped. With
+// cr, cg, cb the cube roots of the mixed channels, the positive XYB is
+// X = 7 (cr - cg) + 0.42, Y = (cr + cg) / 2 - cbrt(bias) + 0.01 and
+// B = cb - (cr + cg) / 2 + 0.55: the bias root cancels in X and B.
+__device__ __forceinline__ void xyb_fast_shifted(float r, float g, float b, float& X, float& Y, float& B) {
+  const float cr = cbrt_fast(fmaf(kM00, r, fmaf(kM01, g, fmaf(kM02, b, kB0))));
+  const float cg = cbrt_fast(fmaf(kM10, r, fmaf(kM11, g, fmaf(kM12, b, kB0))));
+  const float cb = cbrt_fast(fmaf(kM20, r, fmaf(kM21, g, fmaf(kM22, b, kB0))));
+  constexpr float kX0 = 0.42f - 0.5f, kY0 = (0.01f - 0.4375f) - kB0Root, kBs = 0.55f - 0.625f;
+  const float sum = cr + cg;
+  X = fmaf(cr - cg, 7.0f, kX0);
+  Y = fmaf(sum, 0.5f, kY0);
+  B = fmaf(sum, -0.5f, cb) + kBs;
 }
 
 // Storage codecs.
@@ -174,17 +177,44 @@ struct ItemRange {
   }
 };
 
-// Sum over the 64 lanes of a wave in a fixed order with DPP adds (no LDS
-// round trips): butterflies within quads and rows of 16, then the row totals
-// are chained into lane 63, which returns the total.
-__device__ __forceinline__ float wave_sum_dpp(float v) {
-  v += __builtin_amdgcn_update_dpp(0.0f, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x124, 0xF, 0xF, false);  // row_ror:4
-  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x128, 0xF, 0xF, false);  // row_ror:8
-  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
-  v += __builtin_amdgcn_update_dpp(0.0f, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
-  return v;
+// The 18 per-lane sums of a tile, each summed over the 64 lanes of the wave
+// in a fixed order, by halving butterflies: at each step a lane pair keeps
+// one value of each pair of values (the lane's side bit picks which) and
+// adds its partner's copy of it, so the 18 values take 9 + 4 + 2 + 1 + 1
+// exchanges instead of 18 full reductions. Partners: lane ^ 15 within a row
+// (side bit 3), lane ^ 7 within 8 (bit 2), lane ^ 2 (bit 1), lane ^ 1 (bit 0),
+// lane ^ 16 (bit 4); an odd value out at a step is summed over the pair in
+// both lanes. Lane L < 16 ends with sum 8 b0 + 4 b1 + 2 b2 + b3 (bits of
+// L), lane 16 with sum 16 and lane 24 with sum 17; *which is -1 for the
+// other lanes.
+template <int CTRL>
+__device__ __forceinline__ float dpp_pair(float keep, float send) {
+  return keep + __builtin_amdgcn_update_dpp(0.0f, send, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_half(float u, float v, bool side) {
+  return dpp_pair<CTRL>(side ? v : u, side ? u : v);
+}
+__device__ __forceinline__ float wave_sums18(const float (&v)[18], int lane, int* which) {
+  constexpr int kMirror = 0x140, kHalfMirror = 0x141, kXor2 = 0x4E, kXor1 = 0xB1;
+  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8, b4 = lane & 16;
+  float s1[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s1[i] = dpp_half<kMirror>(v[2 * i], v[2 * i + 1], b3);
+  float s2[5];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) s2[k] = dpp_half<kHalfMirror>(s1[2 * k], s1[2 * k + 1], b2);
+  s2[4] = dpp_pair<kHalfMirror>(s1[8], s1[8]);
+  const float s30 = dpp_half<kXor2>(s2[0], s2[1], b1), s31 = dpp_half<kXor2>(s2[2], s2[3], b1);
+  const float s32 = dpp_pair<kXor2>(s2[4], s2[4]);
+  const float s40 = dpp_half<kXor1>(s30, s31, b0), s41 = dpp_pair<kXor1>(s32, s32);
+  // lane ^ 16 (swizzle, bit-mask mode: and 0x1f, xor 0x10), then lane ^ 32
+  const float keep = b4 ? s41 : s40, send = b4 ? s40 : s41;
+  const float s5 = keep + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(send), 0x401F));
+  const float t = s5 + __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, __float_as_int(s5)));
+  *which = lane < 16 ? 8 * (lane & 1) + 4 * ((lane >> 1) & 1) + 2 * ((lane >> 2) & 1) + ((lane >> 3) & 1)
+                     : (lane == 16 ? 16 : (lane == 24 ? 17 : -1));
+  return t;
 }
 
 }  // namespace
@@ -205,21 +235,66 @@ struct TileBytes {
 // in-image pixels (scale 0 of every plane is computed here once, not per tile
 // halo in the tile kernel) and forms the scale-1 mean directly, in the order
 // ((x0,y0) + (x1,y0)) + (x0,y1)) + (x1,y1) of cpu.rs:545-579; only scales
-// >= 1 are kept in LDS (9 KB a block).
+// >= 1 are kept in LDS.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __restrict__ pairs,
                                                            const int* __restrict__ prefix,
                                                            const int* __restrict__ meta) {
   constexpr int T0 = kPyrTile, T1 = T0 / 2;
   static_assert(T1 * T1 == kThreads, "one scale-1 pixel per thread");
-  __shared__ float bufB[6][T1][T1 + 1];          // scales 1, 3, 5
+  __shared__ float bufB[2][6][T1][T1 + 1];       // scales 1, 3, 5 of two items in flight
   __shared__ float bufA[6][T1 / 2][T1 / 2 + 1];  // scales 2, 4
   __shared__ float lut_s[256];  // divergent lookups: LDS, not constant memory
   const int npairs = meta[0], n = prefix[npairs];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int sx = tid % T1, sy = tid / T1;  // this thread's scale-1 pixel in the tile
   for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut_fast[i];
+  fast_barrier();  // the table is complete
   lds_cfloat* lut = (lds_cfloat*)lut_s;
+  int nb = 0;  // items with scale 1 so far: selects the scale-1 buffer and the level wave
+  // Scales 2.. (at most 8x8 pixels) of an item run on one wave after the
+  // next item's barrier, rotating over the items: a wave's LDS accesses take
+  // effect in order, so the levels need no block barrier of their own, and
+  // the other waves meanwhile work on the next item.
+  auto levels = [&](const PairDesc& Q, int tx, int ty, float (*B1)[T1][T1 + 1]) {
+    for (int s = 2; s < Q.nscales; ++s) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const int Ts = T0 >> s;
+      const int pw = Q.sc[s - 1].w, ph = Q.sc[s - 1].h;
+      const int cw = Q.sc[s].w, ch = Q.sc[s].h;
+      const int ox0 = tx * Ts, oy0 = ty * Ts;
+      const int lx = lane % Ts, ly = lane / Ts;
+      const int gx = ox0 + lx, gy = oy0 + ly;
+      if (lane >= Ts * Ts || gx >= cw || gy >= ch) continue;
+      const int x0 = min(2 * gx, pw - 1) - 2 * ox0, x1 = min(2 * gx + 1, pw - 1) - 2 * ox0;
+      const int y0 = min(2 * gy, ph - 1) - 2 * oy0, y1 = min(2 * gy + 1, ph - 1) - 2 * oy0;
+#pragma unroll
+      for (int img = 0; img < 2; ++img) {
+        float rgb[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int k = img * 3 + c;
+          float sum;
+          if (s & 1)
+            sum = ((bufA[k][y0][x0] + bufA[k][y0][x1]) + bufA[k][y1][x0]) + bufA[k][y1][x1];
+          else
+            sum = ((B1[k][y0][x0] + B1[k][y0][x1]) + B1[k][y1][x0]) + B1[k][y1][x1];
+          rgb[c] = sum * 0.25f;
+          if (s & 1)
+            B1[k][ly][lx] = rgb[c];
+          else
+            bufA[k][ly][lx] = rgb[c];
+        }
+        float X, Y, B;
+        xyb_fast_shifted(rgb[0], rgb[1], rgb[2], X, Y, B);
+        const long long off = (long long)gy * Q.sc[s].pitch + gx;
+        fast_plane<T>(Q, s, img, 0)[off] = Codec<T>::enc(X);
+        fast_plane<T>(Q, s, img, 1)[off] = Codec<T>::enc(Y);
+        fast_plane<T>(Q, s, img, 2)[off] = Codec<T>::enc(B);
+      }
+    }
+  };
+  int pend = -1, pend_tx = 0, pend_ty = 0, pend_nb = 0;  // item whose levels are due
   ItemRange items(prefix, npairs, n);
   // Scale-0 coordinates of the 2x2 block, clamped into the image.
   auto block_xy = [&](const PairDesc& Q, int local, int k, int& gx, int& gy) {
@@ -229,7 +304,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     gy = min(by + 2 * sy + (k >> 1), Q.h - 1);
   };
   // Input bytes of sRGB u8 tiles are fetched one item ahead, so their
-  // latency overlaps the previous tile's pyramid levels.
+  // latency overlaps the previous tile's work.
   auto fetch = [&](int j, int pj, TileBytes& tb) {
     const PairDesc& Q = pairs[pj];
     if (Q.in_kind != kInputSrgbU8) return;
@@ -257,10 +332,17 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     const int local = idx - prefix[pi];
     const int ntx = (P.w + T0 - 1) / T0;
     const int tx = local % ntx, ty = local / ntx;
-    // Linear RGB of this thread's 2x2 block in both images (the branch on
-    // the input kind is uniform over the block).
+    const bool u8 = P.in_kind == kInputSrgbU8;  // uniform over the block
+    // The one barrier of an item: the previous item's scale 1 is in LDS, and
+    // every wave is done with the scale-1 buffer of the item before that.
+    fast_barrier();
+    if (pend >= 0) {
+      if (wv == (pend_nb & 3)) levels(pairs[pend], pend_tx, pend_ty, bufB[pend_nb & 1]);
+      pend = -1;
+    }
+    // Linear RGB of this thread's 2x2 block in both images.
     float px[2][4][3];
-    if (P.in_kind == kInputSrgbU8) {
+    if (u8) {
       // the fetched bytes are the last loads issued; stores issued since
       // may complete before them, so only a full wait proves them done
       __builtin_amdgcn_s_waitcnt(vmcnt_only<0>());
@@ -299,11 +381,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
       float v[4][3];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        float X, Y, B;
-        xyb_fast(px[img][k][0], px[img][k][1], px[img][k][2], X, Y, B);
-        v[k][0] = X - xyb_shift(0);
-        v[k][1] = Y - xyb_shift(1);
-        v[k][2] = B - xyb_shift(2);
+        xyb_fast_shifted(px[img][k][0], px[img][k][1], px[img][k][2], v[k][0], v[k][1], v[k][2]);
       }
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
@@ -323,61 +401,34 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
       for (int c = 0; c < 3; ++c)
         lin[img][c] = (((px[img][0][c] + px[img][1][c]) + px[img][2][c]) + px[img][3][c]) * 0.25f;
     if (P.nscales < 2) continue;  // uniform: the whole block has this item
-    fast_barrier();  // the previous item's LDS reads are done
-    {
-      const int gx = tx * T1 + sx, gy = ty * T1 + sy;
+    // Scale 1, one pixel a thread, into the scale-1 buffer of this item's
+    // parity (the other may be in use by the previous item's levels).
+    float (*B1)[T1][T1 + 1] = bufB[nb & 1];
+    const int gx = tx * T1 + sx, gy = ty * T1 + sy;
 #pragma unroll
-      for (int img = 0; img < 2; ++img) {
+    for (int img = 0; img < 2; ++img) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bufB[img * 3 + c][sy][sx] = lin[img][c];
-        if (gx < P.sc[1].w && gy < P.sc[1].h) {
-          float X, Y, B;
-          xyb_fast(lin[img][0], lin[img][1], lin[img][2], X, Y, B);
-          const long long off = (long long)gy * P.sc[1].pitch + gx;
-          fast_plane<T>(P, 1, img, 0)[off] = Codec<T>::enc(X - xyb_shift(0));
-          fast_plane<T>(P, 1, img, 1)[off] = Codec<T>::enc(Y - xyb_shift(1));
-          fast_plane<T>(P, 1, img, 2)[off] = Codec<T>::enc(B - xyb_shift(2));
-        }
+      for (int c = 0; c < 3; ++c) B1[img * 3 + c][sy][sx] = lin[img][c];
+      if (gx < P.sc[1].w && gy < P.sc[1].h) {
+        float X, Y, B;
+        xyb_fast_shifted(lin[img][0], lin[img][1], lin[img][2], X, Y, B);
+        const long long off = (long long)gy * P.sc[1].pitch + gx;
+        fast_plane<T>(P, 1, img, 0)[off] = Codec<T>::enc(X);
+        fast_plane<T>(P, 1, img, 1)[off] = Codec<T>::enc(Y);
+        fast_plane<T>(P, 1, img, 2)[off] = Codec<T>::enc(B);
       }
     }
-    for (int s = 2; s < P.nscales; ++s) {
-      fast_barrier();
-      const int Ts = T0 >> s;
-      const int pw = P.sc[s - 1].w, ph = P.sc[s - 1].h;
-      const int cw = P.sc[s].w, ch = P.sc[s].h;
-      const int ox0 = tx * Ts, oy0 = ty * Ts;
-      for (int p = tid; p < Ts * Ts; p += kThreads) {
-        const int lx = p % Ts, ly = p / Ts;
-        const int gx = ox0 + lx, gy = oy0 + ly;
-        if (gx >= cw || gy >= ch) continue;
-        const int x0 = min(2 * gx, pw - 1) - 2 * ox0, x1 = min(2 * gx + 1, pw - 1) - 2 * ox0;
-        const int y0 = min(2 * gy, ph - 1) - 2 * oy0, y1 = min(2 * gy + 1, ph - 1) - 2 * oy0;
-#pragma unroll
-        for (int img = 0; img < 2; ++img) {
-          float rgb[3];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const int k = img * 3 + c;
-            float sum;
-            if (s & 1)
-              sum = ((bufA[k][y0][x0] + bufA[k][y0][x1]) + bufA[k][y1][x0]) + bufA[k][y1][x1];
-            else
-              sum = ((bufB[k][y0][x0] + bufB[k][y0][x1]) + bufB[k][y1][x0]) + bufB[k][y1][x1];
-            rgb[c] = sum * 0.25f;
-            if (s & 1)
-              bufB[k][ly][lx] = rgb[c];
-            else
-              bufA[k][ly][lx] = rgb[c];
-          }
-          float X, Y, B;
-          xyb_fast(rgb[0], rgb[1], rgb[2], X, Y, B);
-          const long long off = (long long)gy * P.sc[s].pitch + gx;
-          fast_plane<T>(P, s, img, 0)[off] = Codec<T>::enc(X - xyb_shift(0));
-          fast_plane<T>(P, s, img, 1)[off] = Codec<T>::enc(Y - xyb_shift(1));
-          fast_plane<T>(P, s, img, 2)[off] = Codec<T>::enc(B - xyb_shift(2));
-        }
-      }
+    if (P.nscales > 2) {
+      pend = pi;
+      pend_tx = tx;
+      pend_ty = ty;
+      pend_nb = nb;
     }
+    ++nb;
+  }
+  if (pend >= 0) {  // uniform: the last item's levels
+    fast_barrier();
+    if (wv == (pend_nb & 3)) levels(pairs[pend], pend_tx, pend_ty, bufB[pend_nb & 1]);
   }
 }
 
@@ -593,15 +644,18 @@ void k_fast_tile(const PairDesc* __restrict__ pairs,
       else
         maps(std::true_type{});
     }
-    // Block sums in a fixed order: f32 within each wave (DPP), then f64 over
-    // the waves in sequence.
+    // Block sums in a fixed order: f32 within each wave, then f64 over the
+    // waves in sequence.
+    {
+      float v18[18];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        const float v = wave_sum_dpp(acc[c][j]);
-        if (lane == 63) red[wv][c * 6 + j] = (double)v;
-      }
+        for (int j = 0; j < 6; ++j) v18[c * 6 + j] = acc[c][j];
+      int which;
+      const float v = wave_sums18(v18, lane, &which);
+      if (which >= 0) red[wv][which] = (double)v;
+    }
     fast_barrier();
     if (tid < 18) {
       double v = 0.0;
