@@ -143,7 +143,7 @@ def test_autotune_keeps_scores(s2, mix):
         rep = b.autotune(dev, reps=2)
         _say(rep)
         cfg, mult = b.tiling()
-        assert 0 <= cfg < 4 and mult in (4, 8, 16)
+        assert 0 <= cfg < 4 and mult in (16, 32, 64, 128, 256)
         after = b.compute(dev)
         for t in range(4):
             b.set_tiling(t, 2)

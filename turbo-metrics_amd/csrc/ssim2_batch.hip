@@ -83,7 +83,7 @@ bool plan_fast(int w, int h, int storage, int cfg, size_t arena_off, int part_ba
 struct ssim2_batch {
   int storage = kStorageF32;
   int tile_cfg = 0;
-  int grid_mult = 8;
+  int grid_mult = 64;  // autotuned batches of a few hundred pairs pick 64..256
   uint32_t max_pairs = 0;
   uint64_t max_pixels = 0;
   size_t arena_cap = 0, part_cap = 0;
@@ -442,8 +442,8 @@ ssim2_status ssim2_batch_compute_sync(ssim2_batch* b, const ssim2_pair* pairs, u
 }
 
 ssim2_status ssim2_batch_set_tiling(ssim2_batch* b, uint32_t tile_cfg, uint32_t grid_mult) {
-  if (!b || tile_cfg >= (uint32_t)kNumTileCfgs || grid_mult == 0 || grid_mult > 64)
-    return bfail(SSIM2_INVALID_ARGUMENT, "tile_cfg in [0, 3], grid_mult in [1, 64]");
+  if (!b || tile_cfg >= (uint32_t)kNumTileCfgs || grid_mult == 0 || grid_mult > 1024)
+    return bfail(SSIM2_INVALID_ARGUMENT, "tile_cfg in [0, 3], grid_mult in [1, 1024]");
   b->tile_cfg = (int)tile_cfg;
   b->grid_mult = (int)grid_mult;
   return SSIM2_OK;
@@ -464,7 +464,7 @@ ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint3
   hipEvent_t e0, e1;
   BHIP(hipEventCreate(&e0));
   BHIP(hipEventCreate(&e1));
-  static const int kMults[] = {4, 8, 16};
+  static const int kMults[] = {16, 32, 64, 128, 256};
   float best = 1e30f;
   int best_cfg = b->tile_cfg, best_mult = b->grid_mult;
   std::string rep = "tile     grid_x_cu  ms/batch\n";

@@ -219,7 +219,8 @@ ssim2_status ssim2_server_create(const ssim2_server_options* o, ssim2_server** o
       delete sv;
       return sfail(st != SSIM2_OK ? st : SSIM2_HIP_ERROR, "server slot: " + m);
     }
-    if (o->tile_cfg || o->grid_mult) ssim2_batch_set_tiling(s.batch, o->tile_cfg, o->grid_mult ? o->grid_mult : 8);
+    // serving batches are small: a smaller grid unless the caller sets one
+    ssim2_batch_set_tiling(s.batch, o->tile_cfg, o->grid_mult ? o->grid_mult : 8);
   }
   sv->worker = std::thread([sv] { sv->run(); });
   *out = sv;
