@@ -100,9 +100,14 @@ struct ssim2_batch {
     double* h_results = nullptr;
     hipEvent_t done = nullptr;
     hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_src = nullptr;        // kept for the pyramid node's updates
+    hipGraphNode_t pyr_node = nullptr;     // the pyramid launch in graph_src
+    hipKernelNodeParams pyr_params = {};
     int graph_cfg = -1, graph_mult = -1;
+    int pyr_grid = 0;                      // pyramid grid of the instantiated graph
     bool pending = false;
     int n = 0;
+    int npyr = 0;                          // pyramid items of the staged batch
   } slot[2];
   int cur = 0;     // slot of the last submitted batch
   int staged = 1;  // slot filled by the last stage()
@@ -121,6 +126,7 @@ int* tile_prefix(ssim2_batch* b, int* base) { return base + 4 + (b->max_pairs + 
 void release(ssim2_batch* b) {
   for (ssim2_batch::Slot& S : b->slot) {
     if (S.graph) (void)hipGraphExecDestroy(S.graph);
+    if (S.graph_src) (void)hipGraphDestroy(S.graph_src);
     if (S.done) (void)hipEventDestroy(S.done);
     if (S.results) (void)hipFree(S.results);
     if (S.d_desc) (void)hipFree(S.d_desc);
@@ -136,38 +142,70 @@ void release(ssim2_batch* b) {
 
 ssim2_status enqueue_kernels(ssim2_batch* b, const ssim2_batch::Slot& S, hipStream_t s, int cfg, int mult) {
   const int* meta = meta_ptr(S.d_index);
-  BHIP(launch_fast_pyramid(b->storage, S.d_desc, pyr_prefix(b, S.d_index), meta, b->cus, s));
+  BHIP(launch_fast_pyramid(b->storage, S.d_desc, pyr_prefix(b, S.d_index), meta, S.npyr, s));
   BHIP(launch_fast_tile(b->storage, cfg, S.d_desc, tile_prefix(b, S.d_index), meta, b->cus * mult, s));
   BHIP(launch_fast_finalize(S.d_desc, meta, b->cus * 4, (int)b->max_pairs, s));
   return SSIM2_OK;
 }
 
-// Capture the three persistent launches once per (tile shape, grid size); the
-// grids are fixed and all sizes come from device memory, so the same graph
-// runs every batch.
+// Capture the three launches once per (tile shape, grid size). All sizes
+// come from device memory, so the same graph runs every batch; only the
+// pyramid's grid (one block per tile) is set per batch, on its node.
 ssim2_status ensure_graph(ssim2_batch* b, ssim2_batch::Slot& S) {
-  if (S.graph && S.graph_cfg == b->tile_cfg && S.graph_mult == b->grid_mult) return SSIM2_OK;
-  if (S.graph) {
-    (void)hipGraphExecDestroy(S.graph);
-    S.graph = nullptr;
+  if (!(S.graph && S.graph_cfg == b->tile_cfg && S.graph_mult == b->grid_mult)) {
+    if (S.graph) {
+      (void)hipGraphExecDestroy(S.graph);
+      S.graph = nullptr;
+    }
+    if (S.graph_src) {
+      (void)hipGraphDestroy(S.graph_src);
+      S.graph_src = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    BHIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    ssim2_status st = enqueue_kernels(b, S, b->cap_stream, b->tile_cfg, b->grid_mult);
+    hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (st != SSIM2_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return bfail(SSIM2_HIP_ERROR, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    // the pyramid's kernel node
+    size_t nn = 0;
+    S.pyr_node = nullptr;
+    if ((e = hipGraphGetNodes(g, nullptr, &nn)) == hipSuccess) {
+      std::vector<hipGraphNode_t> nodes(nn);
+      e = hipGraphGetNodes(g, nodes.data(), &nn);
+      const void* pk = fast_pyramid_kernel(b->storage);
+      for (size_t i = 0; e == hipSuccess && i < nn && !S.pyr_node; ++i) {
+        hipGraphNodeType t;
+        if ((e = hipGraphNodeGetType(nodes[i], &t)) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp;
+        if ((e = hipGraphKernelNodeGetParams(nodes[i], &kp)) == hipSuccess && kp.func == pk) {
+          S.pyr_node = nodes[i];
+          S.pyr_params = kp;
+        }
+      }
+    }
+    if (e == hipSuccess && !S.pyr_node) e = hipErrorNotFound;
+    if (e == hipSuccess) e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      S.graph = nullptr;
+      return bfail(SSIM2_HIP_ERROR, std::string("batch graph: ") + hipGetErrorString(e));
+    }
+    S.graph_src = g;
+    S.graph_cfg = b->tile_cfg;
+    S.graph_mult = b->grid_mult;
+    S.pyr_grid = (int)S.pyr_params.gridDim.x;
   }
-  hipGraph_t g = nullptr;
-  BHIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-  ssim2_status st = enqueue_kernels(b, S, b->cap_stream, b->tile_cfg, b->grid_mult);
-  hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
-  if (st != SSIM2_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return st;
+  const int grid = S.npyr < 1 ? 1 : S.npyr;
+  if (grid != S.pyr_grid) {
+    hipKernelNodeParams kp = S.pyr_params;
+    kp.gridDim = dim3(grid);
+    BHIP(hipGraphExecKernelNodeSetParams(S.graph, S.pyr_node, &kp));
+    S.pyr_grid = grid;
   }
-  if (e != hipSuccess) return bfail(SSIM2_HIP_ERROR, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) {
-    S.graph = nullptr;
-    return bfail(SSIM2_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  }
-  S.graph_cfg = b->tile_cfg;
-  S.graph_mult = b->grid_mult;
   return SSIM2_OK;
 }
 
@@ -256,6 +294,7 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
     return bfail(SSIM2_BATCH_SIZE, "batch of " + std::to_string(px) + " px exceeds the capacity of " +
                                        std::to_string(b->max_pixels) + " px");
   pp[n] = pyr;
+  SL.npyr = pyr;
   tp[n] = tiles;
   int* meta = meta_ptr(SL.h_index);
   meta[0] = (int)n;

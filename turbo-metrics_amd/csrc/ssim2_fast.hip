@@ -668,38 +668,33 @@ void k_fast_tile(const PairDesc* __restrict__ pairs,
 
 // ---------------------------------------------------------------- launchers
 
-// Persistent grid: as many blocks as the CUs hold at once (queried once per
-// storage type), so every block's static item range starts at launch.
-template <typename T>
-static int pyramid_blocks_per_cu() {
-  static int n = 0;
-  if (n == 0) {
-    int v = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, reinterpret_cast<const void*>(&k_fast_pyramid<T>), kThreads,
-                                                     0) != hipSuccess ||
-        v < 1) {
-      (void)hipGetLastError();
-      v = 4;
-    }
-    n = v;
-  }
-  return n;
-}
-
-hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta,
-                               int cus, hipStream_t stream) {
+// One block per tile: blocks resident together start at different times, so
+// their phases (loads, cube roots, stores) overlap instead of running in step
+// as a persistent grid's do. A smaller grid still covers every item (each
+// block takes a contiguous range).
+const void* fast_pyramid_kernel(int storage) {
   switch (storage) {
     case kStorageF32:
-      hipLaunchKernelGGL(k_fast_pyramid<float>, dim3(cus * pyramid_blocks_per_cu<float>()), dim3(kThreads), 0, stream,
-                         pairs, prefix, meta);
+      return reinterpret_cast<const void*>(&k_fast_pyramid<float>);
+    case kStorageF16:
+      return reinterpret_cast<const void*>(&k_fast_pyramid<_Float16>);
+    default:
+      return reinterpret_cast<const void*>(&k_fast_pyramid<bf16_t>);
+  }
+}
+
+hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta, int grid,
+                               hipStream_t stream) {
+  grid = grid < 1 ? 1 : grid;
+  switch (storage) {
+    case kStorageF32:
+      hipLaunchKernelGGL(k_fast_pyramid<float>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
     case kStorageF16:
-      hipLaunchKernelGGL(k_fast_pyramid<_Float16>, dim3(cus * pyramid_blocks_per_cu<_Float16>()), dim3(kThreads), 0,
-                         stream, pairs, prefix, meta);
+      hipLaunchKernelGGL(k_fast_pyramid<_Float16>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
     default:
-      hipLaunchKernelGGL(k_fast_pyramid<bf16_t>, dim3(cus * pyramid_blocks_per_cu<bf16_t>()), dim3(kThreads), 0,
-                         stream, pairs, prefix, meta);
+      hipLaunchKernelGGL(k_fast_pyramid<bf16_t>, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
   }
   return hipGetLastError();

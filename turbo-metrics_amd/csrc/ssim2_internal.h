@@ -145,9 +145,11 @@ hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems,
                        hipStream_t stream);
 hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream);
 // Fast form (ssim2_fast.hip). `prefix` holds npairs+1 first-item indices,
-// `meta[0]` the pair count; grids are fixed (persistent kernels).
-hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta,
-                               int cus, hipStream_t stream);  // grid: cus x resident blocks per CU
+// `meta[0]` the pair count; every grid covers all items of a batch whatever
+// its size (a block takes a contiguous range of items).
+hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* prefix, const int* meta, int grid,
+                               hipStream_t stream);  // grid: best one block per tile
+const void* fast_pyramid_kernel(int storage);     // the kernel launch_fast_pyramid runs
 hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, const int* prefix,
                             const int* meta, int grid, hipStream_t stream);
 hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, int max_pairs,
