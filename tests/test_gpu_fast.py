@@ -143,11 +143,17 @@ def test_autotune_keeps_scores(s2, mix):
         rep = b.autotune(dev, reps=2)
         _say(rep)
         cfg, mult = b.tiling()
-        assert 0 <= cfg < 4 and mult in (16, 32, 64, 128, 256)
+        assert 0 <= cfg < 4 and mult in (32, 64, 128, 256, 0)
         after = b.compute(dev)
         for t in range(4):
             b.set_tiling(t, 2)
             np.testing.assert_allclose(b.compute(dev), before, rtol=0, atol=1e-6)
+        for t in range(4):
+            # one block per tile: the captured graph's grids follow each batch
+            b.set_tiling(t, 0)
+            np.testing.assert_allclose(b.compute(dev), before, rtol=0, atol=1e-6)
+            np.testing.assert_allclose(b.compute(dev[:3]), before[:3], rtol=0, atol=1e-6)
+            np.testing.assert_allclose(b.compute(dev[2:]), before[2:], rtol=0, atol=1e-6)
     np.testing.assert_allclose(after, before, rtol=0, atol=1e-6)
 
 

@@ -100,14 +100,15 @@ struct ssim2_batch {
     double* h_results = nullptr;
     hipEvent_t done = nullptr;
     hipGraphExec_t graph = nullptr;
-    hipGraph_t graph_src = nullptr;        // kept for the pyramid node's updates
+    hipGraph_t graph_src = nullptr;        // kept for the grid updates of two nodes
     hipGraphNode_t pyr_node = nullptr;     // the pyramid launch in graph_src
-    hipKernelNodeParams pyr_params = {};
+    hipGraphNode_t tile_node = nullptr;    // the tile launch
+    hipKernelNodeParams pyr_params = {}, tile_params = {};
     int graph_cfg = -1, graph_mult = -1;
-    int pyr_grid = 0;                      // pyramid grid of the instantiated graph
+    int pyr_grid = 0, tile_grid = 0;       // grids of the instantiated graph
     bool pending = false;
     int n = 0;
-    int npyr = 0;                          // pyramid items of the staged batch
+    int npyr = 0, ntiles = 0;              // items of the staged batch
   } slot[2];
   int cur = 0;     // slot of the last submitted batch
   int staged = 1;  // slot filled by the last stage()
@@ -140,17 +141,23 @@ void release(ssim2_batch* b) {
   if (b->partials) (void)hipFree(b->partials);
 }
 
+// Tile-kernel grid: mult x CUs blocks, or one block per tile for mult 0.
+int tile_grid(const ssim2_batch* b, const ssim2_batch::Slot& S, int mult) {
+  return mult > 0 ? b->cus * mult : (S.ntiles < 1 ? 1 : S.ntiles);
+}
+
 ssim2_status enqueue_kernels(ssim2_batch* b, const ssim2_batch::Slot& S, hipStream_t s, int cfg, int mult) {
   const int* meta = meta_ptr(S.d_index);
   BHIP(launch_fast_pyramid(b->storage, S.d_desc, pyr_prefix(b, S.d_index), meta, S.npyr, s));
-  BHIP(launch_fast_tile(b->storage, cfg, S.d_desc, tile_prefix(b, S.d_index), meta, b->cus * mult, s));
+  BHIP(launch_fast_tile(b->storage, cfg, S.d_desc, tile_prefix(b, S.d_index), meta, tile_grid(b, S, mult), s));
   BHIP(launch_fast_finalize(S.d_desc, meta, b->cus * 4, (int)b->max_pairs, s));
   return SSIM2_OK;
 }
 
 // Capture the three launches once per (tile shape, grid size). All sizes
 // come from device memory, so the same graph runs every batch; only the
-// pyramid's grid (one block per tile) is set per batch, on its node.
+// grids sized by the batch (one block per tile) are set per batch, on their
+// nodes.
 ssim2_status ensure_graph(ssim2_batch* b, ssim2_batch::Slot& S) {
   if (!(S.graph && S.graph_cfg == b->tile_cfg && S.graph_mult == b->grid_mult)) {
     if (S.graph) {
@@ -170,24 +177,29 @@ ssim2_status ensure_graph(ssim2_batch* b, ssim2_batch::Slot& S) {
       return st;
     }
     if (e != hipSuccess) return bfail(SSIM2_HIP_ERROR, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    // the pyramid's kernel node
+    // the pyramid's and the tile kernel's nodes
     size_t nn = 0;
-    S.pyr_node = nullptr;
+    S.pyr_node = S.tile_node = nullptr;
     if ((e = hipGraphGetNodes(g, nullptr, &nn)) == hipSuccess) {
       std::vector<hipGraphNode_t> nodes(nn);
       e = hipGraphGetNodes(g, nodes.data(), &nn);
       const void* pk = fast_pyramid_kernel(b->storage);
-      for (size_t i = 0; e == hipSuccess && i < nn && !S.pyr_node; ++i) {
+      const void* tk = fast_tile_kernel(b->storage, b->tile_cfg);
+      for (size_t i = 0; e == hipSuccess && i < nn; ++i) {
         hipGraphNodeType t;
         if ((e = hipGraphNodeGetType(nodes[i], &t)) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
         hipKernelNodeParams kp;
-        if ((e = hipGraphKernelNodeGetParams(nodes[i], &kp)) == hipSuccess && kp.func == pk) {
+        if ((e = hipGraphKernelNodeGetParams(nodes[i], &kp)) != hipSuccess) break;
+        if (kp.func == pk) {
           S.pyr_node = nodes[i];
           S.pyr_params = kp;
+        } else if (kp.func == tk) {
+          S.tile_node = nodes[i];
+          S.tile_params = kp;
         }
       }
     }
-    if (e == hipSuccess && !S.pyr_node) e = hipErrorNotFound;
+    if (e == hipSuccess && (!S.pyr_node || !S.tile_node)) e = hipErrorNotFound;
     if (e == hipSuccess) e = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
       (void)hipGraphDestroy(g);
@@ -198,6 +210,7 @@ ssim2_status ensure_graph(ssim2_batch* b, ssim2_batch::Slot& S) {
     S.graph_cfg = b->tile_cfg;
     S.graph_mult = b->grid_mult;
     S.pyr_grid = (int)S.pyr_params.gridDim.x;
+    S.tile_grid = (int)S.tile_params.gridDim.x;
   }
   const int grid = S.npyr < 1 ? 1 : S.npyr;
   if (grid != S.pyr_grid) {
@@ -205,6 +218,13 @@ ssim2_status ensure_graph(ssim2_batch* b, ssim2_batch::Slot& S) {
     kp.gridDim = dim3(grid);
     BHIP(hipGraphExecKernelNodeSetParams(S.graph, S.pyr_node, &kp));
     S.pyr_grid = grid;
+  }
+  const int tgrid = tile_grid(b, S, b->grid_mult);
+  if (tgrid != S.tile_grid) {
+    hipKernelNodeParams kp = S.tile_params;
+    kp.gridDim = dim3(tgrid);
+    BHIP(hipGraphExecKernelNodeSetParams(S.graph, S.tile_node, &kp));
+    S.tile_grid = tgrid;
   }
   return SSIM2_OK;
 }
@@ -295,6 +315,7 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
                                        std::to_string(b->max_pixels) + " px");
   pp[n] = pyr;
   SL.npyr = pyr;
+  SL.ntiles = tiles;
   tp[n] = tiles;
   int* meta = meta_ptr(SL.h_index);
   meta[0] = (int)n;
@@ -481,8 +502,8 @@ ssim2_status ssim2_batch_compute_sync(ssim2_batch* b, const ssim2_pair* pairs, u
 }
 
 ssim2_status ssim2_batch_set_tiling(ssim2_batch* b, uint32_t tile_cfg, uint32_t grid_mult) {
-  if (!b || tile_cfg >= (uint32_t)kNumTileCfgs || grid_mult == 0 || grid_mult > 1024)
-    return bfail(SSIM2_INVALID_ARGUMENT, "tile_cfg in [0, 3], grid_mult in [1, 1024]");
+  if (!b || tile_cfg >= (uint32_t)kNumTileCfgs || grid_mult > 1024)
+    return bfail(SSIM2_INVALID_ARGUMENT, "tile_cfg in [0, 3], grid_mult in [0, 1024]");
   b->tile_cfg = (int)tile_cfg;
   b->grid_mult = (int)grid_mult;
   return SSIM2_OK;
@@ -503,7 +524,7 @@ ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint3
   hipEvent_t e0, e1;
   BHIP(hipEventCreate(&e0));
   BHIP(hipEventCreate(&e1));
-  static const int kMults[] = {16, 32, 64, 128, 256};
+  static const int kMults[] = {32, 64, 128, 256, 0};  // 0: one block per tile
   float best = 1e30f;
   int best_cfg = b->tile_cfg, best_mult = b->grid_mult;
   std::string rep = "tile     grid_x_cu  ms/batch\n";
@@ -525,7 +546,10 @@ ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint3
       }
       ms /= reps;
       char line[96];
-      std::snprintf(line, sizeof line, "%2dx%-5d %-10d %.4f\n", kTileCfgW[cfg], kTileCfgH[cfg], mult, ms);
+      if (mult > 0)
+        std::snprintf(line, sizeof line, "%2dx%-5d %-10d %.4f\n", kTileCfgW[cfg], kTileCfgH[cfg], mult, ms);
+      else
+        std::snprintf(line, sizeof line, "%2dx%-5d %-10s %.4f\n", kTileCfgW[cfg], kTileCfgH[cfg], "per tile", ms);
       rep += line;
       if (ms < best) {
         best = ms;
@@ -540,7 +564,11 @@ ssim2_status ssim2_batch_autotune(ssim2_batch* b, const ssim2_pair* pairs, uint3
   b->tile_cfg = best_cfg;
   b->grid_mult = best_mult;
   char line[96];
-  std::snprintf(line, sizeof line, "chosen %dx%d, grid %d x CUs\n", kTileCfgW[best_cfg], kTileCfgH[best_cfg], best_mult);
+  if (best_mult > 0)
+    std::snprintf(line, sizeof line, "chosen %dx%d, grid %d x CUs\n", kTileCfgW[best_cfg], kTileCfgH[best_cfg],
+                  best_mult);
+  else
+    std::snprintf(line, sizeof line, "chosen %dx%d, one block per tile\n", kTileCfgW[best_cfg], kTileCfgH[best_cfg]);
   rep += line;
   b->tune_report = rep;
   if (report && report_size) {

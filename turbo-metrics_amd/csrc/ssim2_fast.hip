@@ -720,8 +720,27 @@ static hipError_t launch_tile_t(int tile_cfg, const PairDesc* pairs, const int* 
   return hipGetLastError();
 }
 
+template <typename T>
+static const void* tile_kernel_t(int tile_cfg) {
+  switch (tile_cfg) {
+    case 0: return reinterpret_cast<const void*>(&k_fast_tile<T, 32, 16, 2>);
+    case 1: return reinterpret_cast<const void*>(&k_fast_tile<T, 64, 16, 2>);
+    case 2: return reinterpret_cast<const void*>(&k_fast_tile<T, 32, 32, 2>);
+    default: return reinterpret_cast<const void*>(&k_fast_tile<T, 32, 24, 3>);
+  }
+}
+
+const void* fast_tile_kernel(int storage, int tile_cfg) {
+  switch (storage) {
+    case kStorageF32: return tile_kernel_t<float>(tile_cfg);
+    case kStorageF16: return tile_kernel_t<_Float16>(tile_cfg);
+    default: return tile_kernel_t<bf16_t>(tile_cfg);
+  }
+}
+
 hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, const int* prefix,
                             const int* meta, int grid, hipStream_t stream) {
+  grid = grid < 1 ? 1 : grid;
   switch (storage) {
     case kStorageF32: return launch_tile_t<float>(tile_cfg, pairs, prefix, meta, grid, stream);
     case kStorageF16: return launch_tile_t<_Float16>(tile_cfg, pairs, prefix, meta, grid, stream);

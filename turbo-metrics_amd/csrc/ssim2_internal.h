@@ -152,6 +152,7 @@ hipError_t launch_fast_pyramid(int storage, const PairDesc* pairs, const int* pr
 const void* fast_pyramid_kernel(int storage);     // the kernel launch_fast_pyramid runs
 hipError_t launch_fast_tile(int storage, int tile_cfg, const PairDesc* pairs, const int* prefix,
                             const int* meta, int grid, hipStream_t stream);
+const void* fast_tile_kernel(int storage, int tile_cfg);  // the kernel launch_fast_tile runs
 hipError_t launch_fast_finalize(const PairDesc* pairs, const int* meta, int grid, int max_pairs,
                                 hipStream_t stream);
 // Shared finalisation (ssim2_finalize.hip): per-(pair, scale, channel)
