@@ -248,8 +248,6 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
   const int npairs = meta[0], n = prefix[npairs];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int sx = tid % T1, sy = tid / T1;  // this thread's scale-1 pixel in the tile
-  for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut_fast[i];
-  fast_barrier();  // the table is complete
   lds_cfloat* lut = (lds_cfloat*)lut_s;
   int nb = 0;  // items with scale 1 so far: selects the scale-1 buffer and the level wave
   // Scales 2.. (at most 8x8 pixels) of an item run on one wave after the
@@ -305,16 +303,20 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
   };
   // Input bytes of sRGB u8 tiles are fetched one item ahead, so their
   // latency overlaps the previous tile's work.
+  // The loads are issued whatever the input kind (other kinds read the
+  // first bytes of the descriptor array, unused), so the count of loads in
+  // flight is the same on every path and later waits can count them.
   auto fetch = [&](int j, int pj, TileBytes& tb) {
     const PairDesc& Q = pairs[pj];
-    if (Q.in_kind != kInputSrgbU8) return;
+    const bool u8 = Q.in_kind == kInputSrgbU8;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       int gx, gy;
       block_xy(Q, j - prefix[pj], k, gx, gy);
 #pragma unroll
       for (int img = 0; img < 2; ++img) {
-        gin_u8* px = (gin_u8*)(img ? Q.in_dis : Q.in_ref) + gy * (img ? Q.in_dis_pitch : Q.in_ref_pitch) + gx * 3;
+        gin_u8* px = u8 ? (gin_u8*)(img ? Q.in_dis : Q.in_ref) + gy * (img ? Q.in_dis_pitch : Q.in_ref_pitch) + gx * 3
+                        : (gin_u8*)pairs;
         __builtin_memcpy(&tb.lo[img][k], (const void*)px, 2);
         tb.hi[img][k] = px[2];
       }
@@ -322,10 +324,16 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
   };
   TileBytes tb;
   int pi_next = 0;  // pair of the item fetched ahead (the cursor runs one item early)
-  if (items.first < items.end) {
-    pi_next = items.pair_of(items.first);
-    fetch(items.first, pi_next, tb);
-  }
+  if (items.first >= items.end) return;  // uniform: a block beyond the last item
+  // The table entry is loaded first and the first item's bytes after it, so
+  // storing the entry waits for its load only and the item's loads stay in
+  // flight across the barrier (which waits for LDS stores).
+  static_assert(kThreads == 256, "one table entry per thread");
+  const float lut_entry = c_lut_fast[tid];
+  pi_next = items.pair_of(items.first);
+  fetch(items.first, pi_next, tb);
+  lut_s[tid] = lut_entry;
+  fast_barrier();  // the table is complete
   for (int idx = items.first; idx < items.end; ++idx) {
     const int pi = pi_next;
     const PairDesc& P = pairs[pi];
