@@ -155,6 +155,17 @@ __device__ __forceinline__ int find_pair(const int* __restrict__ prefix, int npa
   return lo;
 }
 
+// Workgroups go to the 8 XCDs round-robin by id. This renumbering gives the
+// blocks of each XCD one contiguous run of ids (a bijection on [0, grid)),
+// so neighbouring items, whose halo rows and output lines overlap, meet in
+// the same L2 instead of being fetched (and partially written) by several.
+__device__ __forceinline__ int xcd_grouped_block() {
+  constexpr int kXcds = 8;
+  const int g = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b % kXcds, q = g / kXcds, r = g % kXcds;
+  return x * q + min(x, r) + b / kXcds;
+}
+
 // A contiguous range of the flat item list per block (equal-work items), and
 // the owning pair tracked by a cursor that only moves forward: one prefix load
 // per pair boundary instead of a binary search per item.
@@ -163,7 +174,7 @@ struct ItemRange {
   int first, end, pair, next;
   __device__ ItemRange(const int* __restrict__ pre, int npairs, int n) : prefix(pre) {
     const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
-    first = min(n, (int)blockIdx.x * per);
+    first = min(n, xcd_grouped_block() * per);
     end = min(n, first + per);
     pair = first < n ? find_pair(pre, npairs, first) : 0;
     next = first < n ? pre[pair + 1] : 0;
@@ -446,17 +457,16 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
 // One channel at a time is in LDS (a quarter of the block's LDS for all
 // three); the next channel's halo is loaded into registers, four elements
 // a load, while the current one is filtered.
-#ifndef SSIM2_TILE_WAVES_32X16
-#define SSIM2_TILE_WAVES_32X16 6
-#endif
-// Waves per SIMD the registers of a tile shape are sized for (1: no bound).
-template <int TX, int TY>
+// Waves per SIMD the registers of a tile shape are sized for (1: no bound):
+// 6 (80 VGPRs) for 32x16 with 16-bit planes; f32 planes need a few more
+// registers (their 4-wide loads are twice as wide) and would spill at 80.
+template <typename T, int TX, int TY>
 constexpr int tile_waves() {
-  return TX == 32 && TY == 16 ? SSIM2_TILE_WAVES_32X16 : 1;
+  return TX == 32 && TY == 16 && sizeof(T) == 2 ? 6 : 1;
 }
 
 template <typename T, int TX, int TY, int RU>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_waves<TX, TY>())))
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(tile_waves<T, TX, TY>())))
 void k_fast_tile(const PairDesc* __restrict__ pairs,
                                                         const int* __restrict__ prefix,
                                                         const int* __restrict__ meta) {
