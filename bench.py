@@ -223,7 +223,9 @@ def run_mixed(s2, workloads, pool_dev, spec, cls, seconds=1.0, rate=2000.0, reps
     """Throughput on the mixed-size request stream (the first `seconds` of the
     Poisson schedule, all requests present at once): the plain exact f32 path
     pair by pair (one context per size, async on one stream) against the
-    batched fast form with up to 512 pairs / 64 Mpx per launch."""
+    batched fast form with up to 512 pairs / 64 Mpx per launch (two batch
+    engines alternating, each batch's scores read after the next is
+    launched)."""
     per_class = [cls.count(c) for c in range(4)]
     _, pick = workloads.serving_schedule(rate, seconds, pool_per_class=per_class)
     reqs = [int(k) for k in pick]
@@ -257,17 +259,27 @@ def run_mixed(s2, workloads, pool_dev, spec, cls, seconds=1.0, rate=2000.0, reps
         cur_px += p
     if cur:
         batches.append(cur)
-    b = s2.Ssim2Batch(max_pairs=512, max_pixels=64 << 20, storage=storage)
-    b.autotune([pool_dev[k] for k in batches[0]], reps=2, stream=stream)
-    for bt in batches:
-        b.compute([pool_dev[k] for k in bt], stream)
+    # two engines on two streams, alternating: batch k+1 is staged and
+    # launched before batch k's scores are read, as a throughput client would
+    engines = [s2.Ssim2Batch(max_pairs=512, max_pixels=64 << 20, storage=storage) for _ in range(2)]
+    streams = [stream, s2.Stream()]
+    engines[0].autotune([pool_dev[k] for k in batches[0]], reps=2, stream=stream)
+    engines[1].set_tiling(*engines[0].tiling())
+    for i, bt in enumerate(batches):
+        engines[i % 2].compute([pool_dev[k] for k in bt], streams[i % 2])
     t0 = time.perf_counter()
     for _ in range(reps):
-        for bt in batches:
-            b.compute_async([pool_dev[k] for k in bt], stream)
-            b.get_scores()
+        pending = None
+        for i, bt in enumerate(batches):
+            e = engines[i % 2]
+            e.compute_async([pool_dev[k] for k in bt], streams[i % 2])
+            if pending is not None:
+                pending.get_scores()
+            pending = e
+        pending.get_scores()
     dt_batch = (time.perf_counter() - t0) / reps
-    b.close()
+    for e in engines:
+        e.close()
     plain = px / 1e6 / dt_plain
     batched = px / 1e6 / dt_batch
     return {"requests": len(reqs), "mpx": round(px / 1e6, 1), "batches": len(batches),
@@ -425,7 +437,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=["all", "plain", "batch", "serving", "video"], default="all")
+    ap.add_argument("--workload", choices=["all", "plain", "batch", "serving", "video", "mixed"], default="all")
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--serving-rate", type=float, default=2000.0)
     ap.add_argument("--batch-reps", type=int, default=5)
@@ -461,6 +473,13 @@ def main():
         legs = [run_batch(s2, workloads, st, bhost, bspec, args.batch_reps) for st in ("f16", "f32")]
         print(json.dumps({"metric": "ssimulacra2 Mpx/s, 256 uneven pairs per launch", "n_gpus": 1,
                           "legs": [{k: v for k, v in leg.items() if k != "scores"} for leg in legs]}), flush=True)
+        return
+
+    if args.workload == "mixed":
+        spec, cls = workloads.serving_pool_spec()
+        pool_dev = [(s2.DeviceImage.from_host(r), s2.DeviceImage.from_host(d)) for r, d in workloads.generate(spec)]
+        mixed = run_mixed(s2, workloads, pool_dev, spec, cls)
+        print(json.dumps({"metric": "ssimulacra2 Mpx/s on the mixed-size stream", "n_gpus": 1, **mixed}), flush=True)
         return
 
     if args.workload == "video":

@@ -312,8 +312,8 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     gx = min(bx + 2 * sx + (k & 1), Q.w - 1);
     gy = min(by + 2 * sy + (k >> 1), Q.h - 1);
   };
-  // Input bytes of sRGB u8 tiles are fetched one item ahead, so their
-  // latency overlaps the previous tile's work.
+  // Input bytes of an item's sRGB u8 blocks, fetched before its barrier (and
+  // the previous item's levels) so that their latency overlaps them.
   // The loads are issued whatever the input kind (other kinds read the
   // first bytes of the descriptor array, unused), so the count of loads in
   // flight is the same on every path and later waits can count them.
@@ -334,36 +334,39 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     }
   };
   TileBytes tb;
-  int pi_next = 0;  // pair of the item fetched ahead (the cursor runs one item early)
   if (items.first >= items.end) return;  // uniform: a block beyond the last item
   // The table entry is loaded first and the first item's bytes after it, so
   // storing the entry waits for its load only and the item's loads stay in
   // flight across the barrier (which waits for LDS stores).
   static_assert(kThreads == 256, "one table entry per thread");
   const float lut_entry = c_lut_fast[tid];
-  pi_next = items.pair_of(items.first);
-  fetch(items.first, pi_next, tb);
+  int pi = items.pair_of(items.first);
+  fetch(items.first, pi, tb);
   lut_s[tid] = lut_entry;
   fast_barrier();  // the table is complete
   for (int idx = items.first; idx < items.end; ++idx) {
-    const int pi = pi_next;
+    if (idx > items.first) {
+      pi = items.pair_of(idx);
+      fetch(idx, pi, tb);  // in flight across the barrier and the levels
+      // The one barrier of a later item: the previous item's scale 1 is in
+      // LDS, and every wave is done with the scale-1 buffer of the item
+      // before that.
+      fast_barrier();
+      if (pend >= 0) {
+        if (wv == (pend_nb & 3)) levels(pairs[pend], pend_tx, pend_ty, bufB[pend_nb & 1]);
+        pend = -1;
+      }
+    }
     const PairDesc& P = pairs[pi];
     const int local = idx - prefix[pi];
     const int ntx = (P.w + T0 - 1) / T0;
     const int tx = local % ntx, ty = local / ntx;
     const bool u8 = P.in_kind == kInputSrgbU8;  // uniform over the block
-    // The one barrier of an item: the previous item's scale 1 is in LDS, and
-    // every wave is done with the scale-1 buffer of the item before that.
-    fast_barrier();
-    if (pend >= 0) {
-      if (wv == (pend_nb & 3)) levels(pairs[pend], pend_tx, pend_ty, bufB[pend_nb & 1]);
-      pend = -1;
-    }
     // Linear RGB of this thread's 2x2 block in both images.
     float px[2][4][3];
     if (u8) {
-      // the fetched bytes are the last loads issued; stores issued since
-      // may complete before them, so only a full wait proves them done
+      // the fetched bytes are the last loads issued; stores issued before
+      // them may still be in flight, and only a full wait proves them done
       __builtin_amdgcn_s_waitcnt(vmcnt_only<0>());
 #pragma unroll
       for (int img = 0; img < 2; ++img)
@@ -383,10 +386,6 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
           block_xy(P, local, k, gx, gy);
           read_linear(input_of(P, img), gx, gy, lut, px[img][k]);
         }
-    }
-    if (idx + 1 < items.end) {
-      pi_next = items.pair_of(idx + 1);
-      fetch(idx + 1, pi_next, tb);
     }
     // Scale 0: shifted XYB of the block's in-image rows, each row of the
     // block as one two-element store. ux is even and so is the pitch, so the
