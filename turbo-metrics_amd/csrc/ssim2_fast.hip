@@ -638,7 +638,7 @@ void k_fast_tile(const PairDesc* __restrict__ pairs,
             const float dd = fmaxf(1.0f - (nm == den ? 1.0f : nm * __builtin_amdgcn_rcpf(den)), 0.0f);
             const float d2 = dd * dd;
             acc[c][0] += dd;
-            acc[c][1] += d2 * d2;
+            acc[c][1] = fmaf(d2, d2, acc[c][1]);
             const float src = xs[0][oy0 + j + 4][ox + 4], dis = xs[1][oy0 + j + 4][ox + 4];
             float rn = dis - m2, rd = src - m1;
             if constexpr (kEdge) {
@@ -650,9 +650,9 @@ void k_fast_tile(const PairDesc* __restrict__ pairs,
             const float art = fmaxf(d1, 0.0f), det = fmaxf(-d1, 0.0f);
             const float a2 = art * art, e2 = det * det;
             acc[c][2] += art;
-            acc[c][3] += a2 * a2;
+            acc[c][3] = fmaf(a2, a2, acc[c][3]);
             acc[c][4] += det;
-            acc[c][5] += e2 * e2;
+            acc[c][5] = fmaf(e2, e2, acc[c][5]);
           }
         }
       };
