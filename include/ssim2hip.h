@@ -225,6 +225,25 @@ ssim2_status ssim2_batch_mem_usage(const ssim2_batch* batch, size_t* bytes);
  * batch, on any stream. Scores and sums refer to the last batch enqueued. */
 ssim2_status ssim2_batch_compute_async(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
                                        ssim2_stream stream);
+/* Resident reference pyramids for the batched form: the batched counterpart
+ * of the reference's cached reference (crates/ssimulacra2-cuda/src/lib.rs:440-860,
+ * set_reference / compute_with_reference), for callers that score many
+ * distorted images against one reference (an encoder's rate-distortion loop).
+ * ssim2_batch_add_reference computes the reference's pyramid once (it
+ * allocates, and waits on `stream`) and returns an id; pairs naming the id
+ * then skip the reference side of the pyramid. A pair scores the same as the
+ * pair with the reference image given directly. Removing a reference waits
+ * for the batches in flight. */
+typedef struct {
+  uint32_t ref_id;   /* from ssim2_batch_add_reference */
+  uint32_t reserved; /* 0 */
+  ssim2_image dis;   /* packed sRGB u8, the reference's size */
+} ssim2_ref_pair;
+ssim2_status ssim2_batch_add_reference(ssim2_batch* batch, const ssim2_image* ref, ssim2_stream stream,
+                                       uint32_t* ref_id);
+ssim2_status ssim2_batch_remove_reference(ssim2_batch* batch, uint32_t ref_id);
+ssim2_status ssim2_batch_compute_ref_async(ssim2_batch* batch, const ssim2_ref_pair* pairs, uint32_t npairs,
+                                           ssim2_stream stream);
 /* As ssim2_batch_compute_async for YUV 4:2:0 frame pairs (fused ingest). */
 typedef struct {
   ssim2_yuv_image ref;

@@ -107,6 +107,38 @@ def test_back_to_back_batches(s2, mix):
         st2.sync()
 
 
+@pytest.mark.parametrize("storage", ["f16", "f32"])
+def test_resident_references(s2, synth, mix, storage):
+    """A pair naming a resident reference pyramid scores bit-identically to
+    the pair with the reference image itself, in any mix."""
+    _, dev = mix
+    r, d1 = synth.pair(2, 5, 300, 200, 8)
+    r2, d2 = synth.pair(2, 5, 300, 200, 24)
+    assert np.array_equal(r, r2) and not np.array_equal(d1, d2)
+    R, D1, D2 = (s2.DeviceImage.from_host(x) for x in (r, d1, d2))
+    _say(f"[gpu] resident references, {storage}")
+    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000, storage=storage) as b:
+        direct = b.compute([(R, D1), (R, D2)])
+        before = b.mem_usage()
+        rid = b.add_reference(R)
+        assert b.mem_usage() > before
+        got = b.compute_ref([(rid, D1), (rid, D2), (rid, D1)])
+        np.testing.assert_array_equal(got, [direct[0], direct[1], direct[0]])
+        # every pair of the mixed-size set through its own resident reference
+        mixed = b.compute(dev)
+        ids = [b.add_reference(a) for a, _ in dev]
+        np.testing.assert_array_equal(b.compute_ref([(i, d) for i, (_, d) in zip(ids, dev)]), mixed)
+        with pytest.raises(s2.Ssim2Error) as ei:
+            b.compute_ref([(rid, dev[0][1])] if dev[0][1].width != 300 else [(ids[1], D1)])
+        assert ei.value.status == "SSIM2_DIM_MISMATCH"
+        b.remove_reference(rid)
+        with pytest.raises(s2.Ssim2Error) as ei:
+            b.compute_ref([(rid, D1)])
+        assert ei.value.status == "SSIM2_NO_REFERENCE"
+        # the other references are untouched
+        np.testing.assert_array_equal(b.compute_ref([(ids[0], dev[0][1])]), mixed[:1])
+
+
 def test_identical_pairs_score_100(s2, synth):
     r, _ = synth.pair(2, 7, 200, 120, 16)
     a = s2.DeviceImage.from_host(r)

@@ -116,6 +116,18 @@ struct ssim2_batch {
   int cus = 256;
   size_t bytes = 0;
   std::string tune_report;
+  // Resident reference pyramids (shifted XYB of every scale, storage type),
+  // each its own allocation, named by id.
+  struct Ref {
+    uint32_t id = 0;
+    int w = 0, h = 0;
+    unsigned char* planes = nullptr;
+    long long off[kMaxScales] = {};  // byte offset of each scale's X, Y, B planes
+    size_t bytes = 0;
+  };
+  std::vector<Ref> refs;
+  uint32_t next_ref = 1;
+  unsigned char* d_reg = nullptr;  // registration launch: one PairDesc + its item index
 };
 
 namespace {
@@ -137,6 +149,10 @@ void release(ssim2_batch* b) {
     if (S.h_results) (void)hipHostFree(S.h_results);
   }
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  for (ssim2_batch::Ref& r : b->refs)
+    if (r.planes) (void)hipFree(r.planes);
+  b->refs.clear();
+  if (b->d_reg) (void)hipFree(b->d_reg);
   if (b->arena) (void)hipFree(b->arena);
   if (b->partials) (void)hipFree(b->partials);
 }
@@ -229,8 +245,15 @@ ssim2_status ensure_graph(ssim2_batch* b, ssim2_batch::Slot& S) {
   return SSIM2_OK;
 }
 
+const ssim2_batch::Ref* find_ref(const ssim2_batch* b, uint32_t id) {
+  for (const ssim2_batch::Ref& r : b->refs)
+    if (r.id == id) return &r;
+  return nullptr;
+}
+
+// Exactly one of pairs, ypairs, rpairs is non-null.
 ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
-                   const ssim2_yuv_pair* ypairs = nullptr) {
+                   const ssim2_yuv_pair* ypairs = nullptr, const ssim2_ref_pair* rpairs = nullptr) {
   if (n == 0) return bfail(SSIM2_BATCH_SIZE, "empty batch");
   if (n > b->max_pairs)
     return bfail(SSIM2_BATCH_SIZE, std::to_string(n) + " pairs exceed the batch capacity of " +
@@ -249,7 +272,19 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
   int* tp = tile_prefix(b, SL.h_index);
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t qw, qh;
-    if (ypairs) {
+    const ssim2_batch::Ref* ref = nullptr;
+    if (rpairs) {
+      const ssim2_ref_pair& q = rpairs[i];
+      ref = find_ref(b, q.ref_id);
+      if (!ref) return bfail(SSIM2_NO_REFERENCE, "pair " + std::to_string(i) + ": unknown reference id");
+      qw = q.dis.width;
+      qh = q.dis.height;
+      if (!q.dis.ptr) return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": null image");
+      if (int(qw) != ref->w || int(qh) != ref->h)
+        return bfail(SSIM2_DIM_MISMATCH, "pair " + std::to_string(i) + ": reference and distorted sizes differ");
+      if (q.dis.pitch_bytes < size_t(qw) * 3)
+        return bfail(SSIM2_INVALID_ARGUMENT, "pair " + std::to_string(i) + ": pitch smaller than a row");
+    } else if (ypairs) {
       const ssim2_yuv_pair& y = ypairs[i];
       qw = y.ref.width;
       qh = y.ref.height;
@@ -278,7 +313,17 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
                                            std::to_string(qh) + " unsupported");
     px += uint64_t(qw) * qh;
     PairDesc d = fp.d;
-    if (ypairs) {
+    if (rpairs) {
+      // image 0 is the resident pyramid: the pyramid kernel leaves it alone
+      // (its input pointers name the distorted image, never read for it)
+      const ssim2_ref_pair& q = rpairs[i];
+      d.in_ref = d.in_dis = static_cast<const unsigned char*>(q.dis.ptr);
+      d.in_ref_pitch = d.in_dis_pitch = (long long)q.dis.pitch_bytes;
+      d.in_kind = kInputSrgbU8;
+      d.ref_planes = ref->planes;
+      d.skip_img = 1;
+      for (int sc = 0; sc < d.nscales; ++sc) d.sc[sc].cache = ref->off[sc];
+    } else if (ypairs) {
       const ssim2_yuv_pair& y = ypairs[i];
       d.in_ref = static_cast<const unsigned char*>(y.ref.y);
       d.in_dis = static_cast<const unsigned char*>(y.dis.y);
@@ -441,6 +486,84 @@ ssim2_status ssim2_batch_compute_async(ssim2_batch* b, const ssim2_pair* pairs, 
                                        ssim2_stream stream) {
   if (!b || (!pairs && npairs)) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
   ssim2_status st = stage(b, pairs, npairs, b->tile_cfg);
+  if (st != SSIM2_OK) return st;
+  return submit(b, npairs, (hipStream_t)stream, true, b->tile_cfg, b->grid_mult);
+}
+
+ssim2_status ssim2_batch_add_reference(ssim2_batch* b, const ssim2_image* ref, ssim2_stream stream,
+                                       uint32_t* ref_id) {
+  if (!b || !ref || !ref_id || !ref->ptr) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  if (ref->pitch_bytes < size_t(ref->width) * 3) return bfail(SSIM2_INVALID_ARGUMENT, "pitch smaller than a row");
+  FastPlan fp;
+  if (!plan_fast((int)ref->width, (int)ref->height, b->storage, b->tile_cfg, 0, 0, &fp))
+    return bfail(SSIM2_INVALID_SIZE, "size " + std::to_string(ref->width) + "x" + std::to_string(ref->height) +
+                                         " unsupported");
+  ssim2_batch::Ref r;
+  r.w = (int)ref->width;
+  r.h = (int)ref->height;
+  const size_t sb = storage_bytes(b->storage);
+  for (int sc = 0; sc < fp.d.nscales; ++sc) {
+    r.off[sc] = (long long)r.bytes;
+    r.bytes += align256(3 * size_t(fp.d.sc[sc].plane) * sb);
+  }
+  const size_t reg_bytes = align256(sizeof(PairDesc)) + 4 * sizeof(int) + 2 * sizeof(int);
+  hipError_t e = hipSuccess;
+  if (!b->d_reg) e = hipMalloc(&b->d_reg, reg_bytes);
+  if (e == hipSuccess) e = hipMalloc(&r.planes, r.bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return bfail(SSIM2_OOM, std::string("reference pyramid: ") + hipGetErrorString(e));
+  }
+  // One pyramid launch with image 1 left alone writes image 0 (the reference)
+  // into the new planes; the arena is not touched.
+  PairDesc d = fp.d;
+  d.in_ref = d.in_dis = static_cast<const unsigned char*>(ref->ptr);
+  d.in_ref_pitch = d.in_dis_pitch = (long long)ref->pitch_bytes;
+  d.in_kind = kInputSrgbU8;
+  d.arena = reinterpret_cast<float*>(b->arena);
+  d.ref_planes = r.planes;
+  d.skip_img = 2;
+  for (int sc = 0; sc < d.nscales; ++sc) d.sc[sc].cache = r.off[sc];
+  const int index[6] = {1, fp.npyr, 0, 0, 0, fp.npyr};  // meta (4 ints), then prefix {0, npyr}
+  unsigned char* dd = b->d_reg;
+  int* di = reinterpret_cast<int*>(dd + align256(sizeof(PairDesc)));
+  hipStream_t s = (hipStream_t)stream;
+  // the registration buffer is reused: the previous registration's launch
+  // finished (each registration waits for its own below)
+  if ((e = hipMemcpy(dd, &d, sizeof(PairDesc), hipMemcpyHostToDevice)) == hipSuccess &&
+      (e = hipMemcpy(di, index, sizeof(index), hipMemcpyHostToDevice)) == hipSuccess &&
+      (e = launch_fast_pyramid(b->storage, reinterpret_cast<const PairDesc*>(dd), di + 4, di, fp.npyr, s)) ==
+          hipSuccess)
+    e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(r.planes);
+    return bfail(SSIM2_HIP_ERROR, std::string("reference pyramid: ") + hipGetErrorString(e));
+  }
+  r.id = b->next_ref++;
+  b->refs.push_back(r);
+  b->bytes += r.bytes;
+  *ref_id = r.id;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_batch_remove_reference(ssim2_batch* b, uint32_t ref_id) {
+  if (!b) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  for (size_t i = 0; i < b->refs.size(); ++i)
+    if (b->refs[i].id == ref_id) {
+      for (ssim2_batch::Slot& S : b->slot)  // batches in flight may read it
+        if (S.pending) BHIP(hipEventSynchronize(S.done));
+      BHIP(hipFree(b->refs[i].planes));
+      b->bytes -= b->refs[i].bytes;
+      b->refs.erase(b->refs.begin() + (long)i);
+      return SSIM2_OK;
+    }
+  return bfail(SSIM2_NO_REFERENCE, "unknown reference id");
+}
+
+ssim2_status ssim2_batch_compute_ref_async(ssim2_batch* b, const ssim2_ref_pair* pairs, uint32_t npairs,
+                                           ssim2_stream stream) {
+  if (!b || (!pairs && npairs)) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st = stage(b, nullptr, npairs, b->tile_cfg, nullptr, pairs);
   if (st != SSIM2_OK) return st;
   return submit(b, npairs, (hipStream_t)stream, true, b->tile_cfg, b->grid_mult);
 }

@@ -110,6 +110,8 @@ __device__ __forceinline__ void fast_barrier() {
 template <typename T>
 __device__ __forceinline__ __attribute__((address_space(1))) T* fast_plane(const PairDesc& P, int s, int img, int c) {
   const ScaleDesc& S = P.sc[s];
+  if (img == 0 && P.ref_planes)  // a resident reference pyramid (uniform per pair)
+    return (__attribute__((address_space(1))) T*)(reinterpret_cast<T*>(P.ref_planes + S.cache) + (long long)c * S.plane);
   return (__attribute__((address_space(1))) T*)(reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(P.arena) + S.xyb) +
                                                (long long)(img * 3 + c) * S.plane);
 }
@@ -279,6 +281,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
       const int y0 = min(2 * gy, ph - 1) - 2 * oy0, y1 = min(2 * gy + 1, ph - 1) - 2 * oy0;
 #pragma unroll
       for (int img = 0; img < 2; ++img) {
+        if ((Q.skip_img >> img) & 1) continue;
         float rgb[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -326,8 +329,9 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
       block_xy(Q, j - prefix[pj], k, gx, gy);
 #pragma unroll
       for (int img = 0; img < 2; ++img) {
-        gin_u8* px = u8 ? (gin_u8*)(img ? Q.in_dis : Q.in_ref) + gy * (img ? Q.in_dis_pitch : Q.in_ref_pitch) + gx * 3
-                        : (gin_u8*)pairs;
+        const bool used = u8 && !((Q.skip_img >> img) & 1);
+        gin_u8* px = used ? (gin_u8*)(img ? Q.in_dis : Q.in_ref) + gy * (img ? Q.in_dis_pitch : Q.in_ref_pitch) + gx * 3
+                          : (gin_u8*)pairs;
         __builtin_memcpy(&tb.lo[img][k], (const void*)px, 2);
         tb.hi[img][k] = px[2];
       }
@@ -384,7 +388,10 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
         for (int k = 0; k < 4; ++k) {
           int gx, gy;
           block_xy(P, local, k, gx, gy);
-          read_linear(input_of(P, img), gx, gy, lut, px[img][k]);
+          if ((P.skip_img >> img) & 1)
+            px[img][k][0] = px[img][k][1] = px[img][k][2] = 0.0f;
+          else
+            read_linear(input_of(P, img), gx, gy, lut, px[img][k]);
         }
     }
     // Scale 0: shifted XYB of the block's in-image rows, each row of the
@@ -396,6 +403,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     const bool in0 = ux < P.w;
 #pragma unroll
     for (int img = 0; img < 2; ++img) {
+      if ((P.skip_img >> img) & 1) continue;  // uniform: a resident or not-needed pyramid
       float v[4][3];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -425,6 +433,7 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
     const int gx = tx * T1 + sx, gy = ty * T1 + sy;
 #pragma unroll
     for (int img = 0; img < 2; ++img) {
+      if ((P.skip_img >> img) & 1) continue;
 #pragma unroll
       for (int c = 0; c < 3; ++c) B1[img * 3 + c][sy][sx] = lin[img][c];
       if (gx < P.sc[1].w && gy < P.sc[1].h) {

@@ -59,7 +59,8 @@ struct ScaleDesc {
   long long xyb;     // element offset of the 6 XYB planes (ref X,Y,B, dis X,Y,B)
   long long hrow;    // element offset of the 15 row-pass planes (c*5 + q)
   long long cache;   // element offset of the 9 cached-reference planes: ref XYB c, blurred
-                     // ref^2 (sigma11) c, blurred ref (mu1) c, for c = 0..2
+                     // ref^2 (sigma11) c, blurred ref (mu1) c, for c = 0..2; fast form:
+                     // byte offset of this scale's planes in PairDesc::ref_planes
   int part;          // index of the first partial-sum record of this scale
   int tiles_x;       // fast form: output tiles per row of tiles
   int tile0;         // fast form: index of this scale's first tile within the pair
@@ -83,6 +84,10 @@ struct PairDesc {
   int nscales;
   int w, h;
   float* arena;                 // all planes of this pair
+  unsigned char* ref_planes;    // fast form: a resident reference pyramid (its X,Y,B planes of
+                                // scale s at byte sc[s].cache), read in place of image 0's planes
+  int skip_img;                 // fast form: bit i set = the pyramid kernel leaves image i alone
+  int pad_desc;
   double* partials;             // partial-sum records, kNumSums doubles each
   double* result;               // [0] score, [1..108] sums (scale*18 + c*6 + j), [109..120] dims
   ScaleDesc sc[kMaxScales];

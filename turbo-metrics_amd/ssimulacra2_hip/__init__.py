@@ -75,6 +75,10 @@ class _Pair(ctypes.Structure):
     _fields_ = [("ref", _Image), ("dis", _Image)]
 
 
+class _RefPair(ctypes.Structure):
+    _fields_ = [("ref_id", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("dis", _Image)]
+
+
 class _Yuv(ctypes.Structure):
     _fields_ = [
         ("y", ctypes.c_void_p),
@@ -228,6 +232,9 @@ def _load():
         "ssim2_batch_destroy": (vp,),
         "ssim2_batch_mem_usage": (vp, P(sz)),
         "ssim2_batch_compute_async": (vp, P(_Pair), u32, vp),
+        "ssim2_batch_add_reference": (vp, img, vp, P(u32)),
+        "ssim2_batch_remove_reference": (vp, u32),
+        "ssim2_batch_compute_ref_async": (vp, P(_RefPair), u32, vp),
         "ssim2_batch_get_scores": (vp, P(dbl), u32),
         "ssim2_batch_compute_sync": (vp, P(_Pair), u32, vp, P(dbl)),
         "ssim2_batch_query": (vp, P(ctypes.c_int)),
@@ -659,6 +666,30 @@ class Ssim2Batch:
         arr = self._pairs(pairs)
         self._last = (arr, len(pairs))
         _check(lib.ssim2_batch_compute_async(self._b, arr, len(pairs), _stream_ptr(stream)))
+
+    def add_reference(self, ref, stream=None) -> int:
+        """Compute and keep the pyramid of a reference DeviceImage (u8); the
+        id names it in compute_ref pairs (crates/ssimulacra2-cuda/src/lib.rs
+        set_reference, batched)."""
+        v = ctypes.c_uint32()
+        _check(lib.ssim2_batch_add_reference(self._b, ctypes.byref(ref._img), _stream_ptr(stream), ctypes.byref(v)))
+        return v.value
+
+    def remove_reference(self, ref_id: int):
+        _check(lib.ssim2_batch_remove_reference(self._b, ref_id))
+
+    def compute_ref_async(self, pairs, stream=None):
+        """pairs: sequence of (reference id, dis DeviceImage u8)."""
+        arr = (_RefPair * len(pairs))()
+        for i, (rid, d) in enumerate(pairs):
+            arr[i].ref_id = rid
+            arr[i].dis = d._img
+        self._last = (arr, len(pairs))
+        _check(lib.ssim2_batch_compute_ref_async(self._b, arr, len(pairs), _stream_ptr(stream)))
+
+    def compute_ref(self, pairs, stream=None):
+        self.compute_ref_async(pairs, stream)
+        return self.get_scores()
 
     def compute_yuv_async(self, pairs, stream=None):
         """pairs: sequence of (ref DeviceYuv, dis DeviceYuv)."""
