@@ -313,6 +313,17 @@ ssim2_status ssim2_server_destroy(ssim2_server* server);
 /* Thread-safe. SSIM2_ADMISSION_REFUSED when the request can never fit the
  * server's limits; the images must stay valid until its result is polled. */
 ssim2_status ssim2_server_submit(ssim2_server* server, const ssim2_pair* pair, uint64_t request_id);
+/* Resident references of the server (the batched set_reference, see
+ * ssim2_batch_add_reference): the pyramid is computed once and kept in HBM
+ * for every slot; requests naming the id batch with any others and score as
+ * the pair with the reference image would. Registration and removal are
+ * thread-safe and briefly hold the scheduler; removal waits for the batches
+ * in flight, and queued requests naming a removed id fail with
+ * SSIM2_NO_REFERENCE. */
+ssim2_status ssim2_server_add_reference(ssim2_server* server, const ssim2_image* ref, uint32_t* ref_id);
+ssim2_status ssim2_server_remove_reference(ssim2_server* server, uint32_t ref_id);
+ssim2_status ssim2_server_submit_ref(ssim2_server* server, uint32_t ref_id, const ssim2_image* dis,
+                                     uint64_t request_id);
 /* Move up to `capacity` finished results out; *n receives the count. */
 ssim2_status ssim2_server_poll(ssim2_server* server, ssim2_result* out, uint32_t capacity, uint32_t* n);
 /* Wait until every accepted request has finished (timeout_s <= 0: no limit). */

@@ -54,6 +54,44 @@ def test_server_results_equal_alone(s2, pool, storage):
         assert score == alone[order[rid]], (rid, score, alone[order[rid]])
 
 
+@pytest.mark.parametrize("storage", ["f16", "f32"])
+def test_server_resident_references(s2, pool, storage):
+    """Requests naming a resident reference batch with direct ones and score
+    bit-identically to the direct pair."""
+    with s2.Ssim2Batch(max_pairs=8, max_pixels=2_000_000, storage=storage) as b:
+        alone = [b.compute([p])[0] for p in pool]
+    rng = random.Random(11)
+    order = [(rng.randrange(len(pool)), rng.random() < 0.6) for _ in range(80)]
+    _say(f"[gpu] server {storage}: 80 requests, resident references for 60%")
+    with s2.Ssim2Server(storage=storage, slots=3, max_pairs_per_batch=16, max_pixels_per_batch=4_000_000,
+                        hbm_budget=1 << 30) as sv:
+        ids = [sv.add_reference(r) for r, _ in pool]
+        for rid, (k, by_ref) in enumerate(order):
+            if by_ref:
+                sv.submit_ref(ids[k], pool[k][1], rid)
+            else:
+                sv.submit(*pool[k], rid)
+            if rid % 9 == 0:
+                time.sleep(0.0005)
+        sv.drain(60.0)
+        res = sv.poll()
+        assert len(res) == len(order)
+        for rid, score, t0, t1, status in res:
+            assert status == "SSIM2_OK"
+            assert score == alone[order[rid][0]], (rid, score, alone[order[rid][0]])
+        sv.remove_reference(ids[0])
+        with pytest.raises(s2.Ssim2Error) as ei:
+            sv.submit_ref(ids[0], pool[0][1], 1000)
+        assert ei.value.status == "SSIM2_NO_REFERENCE"
+        with pytest.raises(s2.Ssim2Error) as ei:
+            sv.submit_ref(ids[1], pool[0][1], 1001)
+        assert ei.value.status == "SSIM2_DIM_MISMATCH"
+        sv.submit_ref(ids[1], pool[1][1], 1002)  # the others stay resident
+        sv.drain(60.0)
+        (rid, score, _, _, status), = sv.poll()
+        assert rid == 1002 and status == "SSIM2_OK" and score == alone[1]
+
+
 def test_server_admission(s2, pool):
     big = s2.DeviceImage(700, 700)
     with s2.Ssim2Server(storage="f16", slots=1, max_pairs_per_batch=4, max_pixels_per_batch=300_000,
@@ -70,7 +108,7 @@ def test_server_tight_budget_serialises(s2, pool):
         want = b.compute([pool[4]])[0]
     _say("[gpu] server tight budget")
     with s2.Ssim2Server(storage="f16", slots=2, max_pairs_per_batch=4, max_pixels_per_batch=1_000_000,
-                        hbm_budget=2_500_000) as sv:
+                        hbm_budget=5_000_000) as sv:
         for rid in range(6):
             sv.submit(*pool[4], rid)
         sv.drain(60.0)

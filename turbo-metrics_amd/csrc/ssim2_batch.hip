@@ -1,6 +1,7 @@
 // Batch engine of the fast form (include/ssim2hip.h, ssim2_batch_*): many
 // uneven sRGB pairs per launch. Host code: planning, staging, the captured
-// hipGraph of the three persistent kernels, and the tile-shape autotuner.
+// hipGraph of the three kernels, resident reference pyramids, and the
+// tile-shape autotuner.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "../../include/ssim2hip.h"
+#include "ssim2_batch_internal.h"
 #include "ssim2_internal.h"
 #include "ssim2_plan.h"
 
@@ -124,6 +126,7 @@ struct ssim2_batch {
     unsigned char* planes = nullptr;
     long long off[kMaxScales] = {};  // byte offset of each scale's X, Y, B planes
     size_t bytes = 0;
+    bool owned = true;               // false: another engine's planes, shared
   };
   std::vector<Ref> refs;
   uint32_t next_ref = 1;
@@ -150,7 +153,7 @@ void release(ssim2_batch* b) {
   }
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   for (ssim2_batch::Ref& r : b->refs)
-    if (r.planes) (void)hipFree(r.planes);
+    if (r.planes && r.owned) (void)hipFree(r.planes);
   b->refs.clear();
   if (b->d_reg) (void)hipFree(b->d_reg);
   if (b->arena) (void)hipFree(b->arena);
@@ -251,9 +254,11 @@ const ssim2_batch::Ref* find_ref(const ssim2_batch* b, uint32_t id) {
   return nullptr;
 }
 
-// Exactly one of pairs, ypairs, rpairs is non-null.
+// Exactly one of pairs, ypairs, rpairs is non-null. With pairs, ref_ids
+// (optional) names a resident reference per pair (0: the pair's own ref).
 ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
-                   const ssim2_yuv_pair* ypairs = nullptr, const ssim2_ref_pair* rpairs = nullptr) {
+                   const ssim2_yuv_pair* ypairs = nullptr, const ssim2_ref_pair* rpairs = nullptr,
+                   const uint32_t* ref_ids = nullptr) {
   if (n == 0) return bfail(SSIM2_BATCH_SIZE, "empty batch");
   if (n > b->max_pairs)
     return bfail(SSIM2_BATCH_SIZE, std::to_string(n) + " pairs exceed the batch capacity of " +
@@ -273,8 +278,16 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t qw, qh;
     const ssim2_batch::Ref* ref = nullptr;
-    if (rpairs) {
-      const ssim2_ref_pair& q = rpairs[i];
+    ssim2_ref_pair rq{};
+    const bool by_ref = rpairs || (ref_ids && ref_ids[i]);
+    if (by_ref) {
+      if (rpairs) {
+        rq = rpairs[i];
+      } else {
+        rq.ref_id = ref_ids[i];
+        rq.dis = pairs[i].dis;
+      }
+      const ssim2_ref_pair& q = rq;
       ref = find_ref(b, q.ref_id);
       if (!ref) return bfail(SSIM2_NO_REFERENCE, "pair " + std::to_string(i) + ": unknown reference id");
       qw = q.dis.width;
@@ -313,10 +326,10 @@ ssim2_status stage(ssim2_batch* b, const ssim2_pair* pairs, uint32_t n, int cfg,
                                            std::to_string(qh) + " unsupported");
     px += uint64_t(qw) * qh;
     PairDesc d = fp.d;
-    if (rpairs) {
+    if (by_ref) {
       // image 0 is the resident pyramid: the pyramid kernel leaves it alone
       // (its input pointers name the distorted image, never read for it)
-      const ssim2_ref_pair& q = rpairs[i];
+      const ssim2_ref_pair& q = rq;
       d.in_ref = d.in_dis = static_cast<const unsigned char*>(q.dis.ptr);
       d.in_ref_pitch = d.in_dis_pitch = (long long)q.dis.pitch_bytes;
       d.in_kind = kInputSrgbU8;
@@ -552,8 +565,10 @@ ssim2_status ssim2_batch_remove_reference(ssim2_batch* b, uint32_t ref_id) {
     if (b->refs[i].id == ref_id) {
       for (ssim2_batch::Slot& S : b->slot)  // batches in flight may read it
         if (S.pending) BHIP(hipEventSynchronize(S.done));
-      BHIP(hipFree(b->refs[i].planes));
-      b->bytes -= b->refs[i].bytes;
+      if (b->refs[i].owned) {
+        BHIP(hipFree(b->refs[i].planes));
+        b->bytes -= b->refs[i].bytes;
+      }
       b->refs.erase(b->refs.begin() + (long)i);
       return SSIM2_OK;
     }
@@ -567,6 +582,32 @@ ssim2_status ssim2_batch_compute_ref_async(ssim2_batch* b, const ssim2_ref_pair*
   if (st != SSIM2_OK) return st;
   return submit(b, npairs, (hipStream_t)stream, true, b->tile_cfg, b->grid_mult);
 }
+
+}  // extern "C"
+
+namespace ssim2 {
+
+ssim2_status batch_share_reference(ssim2_batch* dst, const ssim2_batch* src, uint32_t ref_id) {
+  const ssim2_batch::Ref* r = find_ref(src, ref_id);
+  if (!dst || !r) return bfail(SSIM2_NO_REFERENCE, "unknown reference id");
+  if (dst->storage != src->storage) return bfail(SSIM2_INVALID_ARGUMENT, "storage types differ");
+  ssim2_batch::Ref c = *r;
+  c.owned = false;
+  dst->refs.push_back(c);
+  return SSIM2_OK;
+}
+
+ssim2_status batch_compute_mixed_async(ssim2_batch* b, const ssim2_pair* pairs, const uint32_t* ref_ids,
+                                       uint32_t npairs, hipStream_t stream) {
+  if (!b || (!pairs && npairs)) return bfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  ssim2_status st = stage(b, pairs, npairs, b->tile_cfg, nullptr, nullptr, ref_ids);
+  if (st != SSIM2_OK) return st;
+  return submit(b, npairs, stream, true, b->tile_cfg, b->grid_mult);
+}
+
+}  // namespace ssim2
+
+extern "C" {
 
 ssim2_status ssim2_batch_compute_yuv_async(ssim2_batch* b, const ssim2_yuv_pair* pairs, uint32_t npairs,
                                            ssim2_stream stream) {

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/ssim2hip.h"
+#include "ssim2_batch_internal.h"
 #include "ssim2_internal.h"
 #include "ssim2_plan.h"
 
@@ -39,8 +40,8 @@ ssim2_status sfail(ssim2_status st, const std::string& m) {
   return st;
 }
 
-// Device bytes a request occupies while in flight in the fast form: XYB of
-// scales >= 1 in the storage type plus its partial-sum records.
+// Device bytes a request occupies while in flight in the fast form: the XYB
+// planes of every scale in the storage type plus its partial-sum records.
 uint64_t request_bytes(uint32_t w, uint32_t h, uint32_t storage) {
   int sw[kMaxScales], sh[kMaxScales];
   const int n = scale_dims((int)w, (int)h, sw, sh);
@@ -48,7 +49,7 @@ uint64_t request_bytes(uint32_t w, uint32_t h, uint32_t storage) {
   uint64_t b = 0;
   for (int s = 0; s < n; ++s) {
     const uint64_t pitch = (uint64_t(sw[s]) + 15) / 16 * 16;
-    if (s > 0) b += 6 * pitch * sh[s] * sb + 256;
+    b += 6 * pitch * sh[s] * sb + 256;
     b += 3 * (uint64_t(sw[s] + 15) / 16) * ((sh[s] + 15) / 16) * kNumSums * sizeof(double);
   }
   return b + kResultDoubles * sizeof(double);
@@ -56,6 +57,7 @@ uint64_t request_bytes(uint32_t w, uint32_t h, uint32_t storage) {
 
 struct Request {
   ssim2_pair pair;
+  uint32_t ref_id = 0;  // a resident reference in place of pair.ref (0: none)
   uint64_t id;
   double t_submit;
   uint64_t px;
@@ -67,6 +69,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   std::vector<Request> reqs;
   std::vector<ssim2_pair> pairs;
+  std::vector<uint32_t> ref_ids;
   std::vector<double> scores;
   uint64_t bytes = 0;
   bool busy = false;
@@ -90,6 +93,16 @@ struct ssim2_server {
   uint64_t n_submitted = 0, n_done = 0, n_refused = 0, n_batches = 0, px_done = 0;
   double busy_since = 0, first_submit = 0, last_done = 0;
   std::string worker_error;
+  // Resident references: computed by slot 0's engine, shared with the
+  // others. ref_mu serialises every batch-engine call of the worker with
+  // reference registration and removal.
+  std::mutex ref_mu;
+  hipStream_t reg_stream = nullptr;
+  struct RefInfo {
+    uint32_t id;
+    uint32_t w, h;
+  };
+  std::vector<RefInfo> refs;
 
   void run();
 };
@@ -102,11 +115,15 @@ void ssim2_server::run() {
     for (Slot& s : slots) {
       if (!s.busy) continue;
       int fin = 0;
-      if (ssim2_batch_query(s.batch, &fin) != SSIM2_OK) fin = 1;
-      if (!fin) continue;
+      ssim2_status st;
+      {
+        std::lock_guard<std::mutex> rk(ref_mu);
+        if (ssim2_batch_query(s.batch, &fin) != SSIM2_OK) fin = 1;
+        if (!fin) continue;
+        s.scores.resize(s.reqs.size());
+        st = ssim2_batch_get_scores(s.batch, s.scores.data(), (uint32_t)s.scores.size());
+      }
       const double t = now_s();
-      s.scores.resize(s.reqs.size());
-      const ssim2_status st = ssim2_batch_get_scores(s.batch, s.scores.data(), (uint32_t)s.scores.size());
       std::lock_guard<std::mutex> lk(mu);
       for (size_t i = 0; i < s.reqs.size(); ++i) {
         ssim2_result r{};
@@ -149,9 +166,18 @@ void ssim2_server::run() {
         in_flight_bytes += bytes;
       }
       s.pairs.resize(s.reqs.size());
-      for (size_t i = 0; i < s.reqs.size(); ++i) s.pairs[i] = s.reqs[i].pair;
+      s.ref_ids.resize(s.reqs.size());
+      for (size_t i = 0; i < s.reqs.size(); ++i) {
+        s.pairs[i] = s.reqs[i].pair;
+        s.ref_ids[i] = s.reqs[i].ref_id;
+      }
       s.t_launch = now_s();
-      const ssim2_status st = ssim2_batch_compute_async(s.batch, s.pairs.data(), (uint32_t)s.pairs.size(), s.stream);
+      ssim2_status st;
+      {
+        std::lock_guard<std::mutex> rk(ref_mu);
+        st = batch_compute_mixed_async(s.batch, s.pairs.data(), s.ref_ids.data(), (uint32_t)s.pairs.size(),
+                                       s.stream);
+      }
       if (st != SSIM2_OK) {
         std::lock_guard<std::mutex> lk(mu);
         worker_error = ssim2_last_error();
@@ -222,9 +248,52 @@ ssim2_status ssim2_server_create(const ssim2_server_options* o, ssim2_server** o
     // serving batches are small: a smaller grid unless the caller sets one
     ssim2_batch_set_tiling(s.batch, o->tile_cfg, o->grid_mult ? o->grid_mult : 8);
   }
+  if (hipStreamCreateWithFlags(&sv->reg_stream, hipStreamNonBlocking) != hipSuccess) {
+    for (Slot& t : sv->slots) {
+      ssim2_batch_destroy(t.batch);
+      (void)hipStreamDestroy(t.stream);
+    }
+    delete sv;
+    return sfail(SSIM2_HIP_ERROR, "server: stream creation failed");
+  }
   sv->worker = std::thread([sv] { sv->run(); });
   *out = sv;
   return SSIM2_OK;
+}
+
+ssim2_status ssim2_server_add_reference(ssim2_server* sv, const ssim2_image* ref, uint32_t* ref_id) {
+  if (!sv || !ref || !ref_id || !ref->ptr) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> rk(sv->ref_mu);
+  uint32_t id = 0;
+  ssim2_status st = ssim2_batch_add_reference(sv->slots[0].batch, ref, sv->reg_stream, &id);
+  if (st != SSIM2_OK) return st;
+  for (size_t i = 1; i < sv->slots.size() && st == SSIM2_OK; ++i)
+    st = batch_share_reference(sv->slots[i].batch, sv->slots[0].batch, id);
+  if (st != SSIM2_OK) {
+    const std::string m = ssim2_last_error();
+    for (size_t i = sv->slots.size(); i-- > 0;) (void)ssim2_batch_remove_reference(sv->slots[i].batch, id);
+    return sfail(st, "server reference: " + m);
+  }
+  sv->refs.push_back({id, ref->width, ref->height});
+  *ref_id = id;
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_server_remove_reference(ssim2_server* sv, uint32_t ref_id) {
+  if (!sv) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> rk(sv->ref_mu);
+  size_t k = 0;
+  while (k < sv->refs.size() && sv->refs[k].id != ref_id) ++k;
+  if (k == sv->refs.size()) return sfail(SSIM2_NO_REFERENCE, "unknown reference id");
+  // the sharing engines first (each waits for its own batches in flight),
+  // then the owner, which frees the planes
+  ssim2_status st = SSIM2_OK;
+  for (size_t i = sv->slots.size(); i-- > 0;) {
+    const ssim2_status e = ssim2_batch_remove_reference(sv->slots[i].batch, ref_id);
+    if (e != SSIM2_OK && st == SSIM2_OK) st = e;
+  }
+  sv->refs.erase(sv->refs.begin() + (long)k);
+  return st;
 }
 
 ssim2_status ssim2_server_submit(ssim2_server* sv, const ssim2_pair* pair, uint64_t request_id) {
@@ -237,6 +306,44 @@ ssim2_status ssim2_server_submit(ssim2_server* sv, const ssim2_pair* pair, uint6
   q.id = request_id;
   q.px = uint64_t(pair->ref.width) * pair->ref.height;
   q.bytes = request_bytes(pair->ref.width, pair->ref.height, sv->opt.storage);
+  if (q.bytes > sv->opt.hbm_budget || q.px > sv->opt.max_pixels_per_batch) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    ++sv->n_refused;
+    return sfail(SSIM2_ADMISSION_REFUSED, "request needs " + std::to_string(q.bytes) + " B / " +
+                                              std::to_string(q.px) + " px, above the server limits");
+  }
+  q.t_submit = now_s();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->n_submitted == 0) sv->first_submit = q.t_submit;
+    ++sv->n_submitted;
+    sv->waiting.push_back(q);
+  }
+  sv->cv.notify_all();
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_server_submit_ref(ssim2_server* sv, uint32_t ref_id, const ssim2_image* dis,
+                                     uint64_t request_id) {
+  if (!sv || !dis || !dis->ptr) return sfail(SSIM2_INVALID_ARGUMENT, "null argument");
+  uint32_t w = 0, h = 0;
+  {
+    std::lock_guard<std::mutex> rk(sv->ref_mu);
+    size_t k = 0;
+    while (k < sv->refs.size() && sv->refs[k].id != ref_id) ++k;
+    if (k == sv->refs.size()) return sfail(SSIM2_NO_REFERENCE, "unknown reference id");
+    w = sv->refs[k].w;
+    h = sv->refs[k].h;
+  }
+  if (dis->width != w || dis->height != h) return sfail(SSIM2_DIM_MISMATCH, "reference and distorted sizes differ");
+  if (dis->pitch_bytes < size_t(w) * 3) return sfail(SSIM2_INVALID_ARGUMENT, "pitch smaller than a row");
+  Request q;
+  q.pair.ref = *dis;  // not read: the resident reference replaces it
+  q.pair.dis = *dis;
+  q.ref_id = ref_id;
+  q.id = request_id;
+  q.px = uint64_t(w) * h;
+  q.bytes = request_bytes(w, h, sv->opt.storage);
   if (q.bytes > sv->opt.hbm_budget || q.px > sv->opt.max_pixels_per_batch) {
     std::lock_guard<std::mutex> lk(sv->mu);
     ++sv->n_refused;
@@ -294,10 +401,13 @@ ssim2_status ssim2_server_destroy(ssim2_server* sv) {
   sv->stop.store(true);
   sv->cv.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
-  for (Slot& s : sv->slots) {
+  // sharing engines first: the owner (slot 0) frees the planes
+  for (size_t i = sv->slots.size(); i-- > 0;) {
+    Slot& s = sv->slots[i];
     if (s.batch) ssim2_batch_destroy(s.batch);
     if (s.stream) (void)hipStreamDestroy(s.stream);
   }
+  if (sv->reg_stream) (void)hipStreamDestroy(sv->reg_stream);
   delete sv;
   return SSIM2_OK;
 }

@@ -245,6 +245,9 @@ def _load():
         "ssim2_server_create": (P(_ServerOptions), P(vp)),
         "ssim2_server_destroy": (vp,),
         "ssim2_server_submit": (vp, P(_Pair), u64),
+        "ssim2_server_add_reference": (vp, img, P(u32)),
+        "ssim2_server_remove_reference": (vp, u32),
+        "ssim2_server_submit_ref": (vp, u32, img, u64),
         "ssim2_server_poll": (vp, P(_Result), u32, P(u32)),
         "ssim2_server_drain": (vp, dbl),
         "ssim2_server_stats": (vp, P(_ServerStats)),
@@ -767,6 +770,19 @@ class Ssim2Server:
         p = _Pair(ref._img, dis._img)
         _check(lib.ssim2_server_submit(self._s, ctypes.byref(p), request_id))
         self._keep[request_id] = (ref, dis)
+
+    def add_reference(self, ref: DeviceImage) -> int:
+        """Keep the pyramid of a reference resident for every slot."""
+        v = ctypes.c_uint32()
+        _check(lib.ssim2_server_add_reference(self._s, ctypes.byref(ref._img), ctypes.byref(v)))
+        return v.value
+
+    def remove_reference(self, ref_id: int):
+        _check(lib.ssim2_server_remove_reference(self._s, ref_id))
+
+    def submit_ref(self, ref_id: int, dis: DeviceImage, request_id: int):
+        _check(lib.ssim2_server_submit_ref(self._s, ref_id, ctypes.byref(dis._img), request_id))
+        self._keep[request_id] = (dis,)
 
     def poll(self):
         """Finished results as a list of (id, score, t_submit, t_done, status)."""
