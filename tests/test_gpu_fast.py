@@ -70,6 +70,33 @@ def test_batch_matches_oracle_and_f64(s2, oracle, mix):
         assert abs(sc - f64) <= BAR_F64, (case, sc, f64)
 
 
+EXTREME = [
+    # thin and odd shapes: tiles cut by both edges at every scale, odd widths
+    # (the pyramid's paired stores), scales below one tile
+    (2, 121, 8, 1000, 16),
+    (4, 122, 1000, 8, 16),
+    (1, 123, 31, 33, 24),
+    (2, 124, 65, 65, 8),
+    (4, 125, 9, 700, 12),
+    (1, 126, 33, 17, 16),
+]
+
+
+def test_extreme_shapes(s2, oracle, synth):
+    host, dev = _upload(s2, synth, EXTREME)
+    _say(f"[gpu] fast batch f32, {len(dev)} thin/odd pairs")
+    with s2.Ssim2Batch(max_pairs=16, max_pixels=1_000_000, storage="f32") as b:
+        scores = b.compute(dev)
+        alone = [b.compute([p])[0] for p in dev]
+    np.testing.assert_array_equal(scores, alone)
+    for (r, d), sc, case in zip(host, scores, EXTREME):
+        same = oracle.score_srgb(r, d, FAST_SHIFT)
+        f64 = oracle.score_srgb(r, d, F64)
+        _say(f"[gpu] {case[2]}x{case[3]}: gpu {sc:.6f} same-arith {same:.6f} f64 {f64:.6f}")
+        assert abs(sc - same) <= TOL_SAME_ARITH, (case, sc, same)
+        assert abs(sc - f64) <= BAR_F64, (case, sc, f64)
+
+
 def test_alone_equals_in_the_mix(s2, mix):
     _, dev = mix
     _say("[gpu] alone vs mix")
