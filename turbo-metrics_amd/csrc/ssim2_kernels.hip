@@ -304,7 +304,15 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 // and drops stores outside the band's rows.
 constexpr int kChunk = 32;                      // columns per staged chunk (8 groups of 4)
 constexpr int kRowPrefetch = 1;                 // chunks of row-pass input in flight per wave
-constexpr int kTileStride = kChunk + 4;         // floats; b128 reads of one row per lane hit distinct banks
+// LDS tiles: rows of 8 float4 (one chunk) with segment g of row r in slot
+// g ^ (r & 7), and the three tiles 3 float4 apart beyond their rows (the
+// first of those is the slot where lanes past a chunk park). LDS serves 16
+// lanes x 16 B a cycle; with this layout the staging and store pattern (two
+// rows of 8 segments per 16 lanes), the scan's reads (five rows, one
+// segment) and most of the term writes fall on distinct banks.
+constexpr int kTileRowF4 = kChunk / 4;
+constexpr int kTileF4 = kRowsPerWave * kTileRowF4 + 3;
+__device__ __forceinline__ int tile_slot(int r, int g) { return r * kTileRowF4 + (g ^ (r & 7)); }
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 constexpr unsigned kRsrcWord3 = 0x00020000u;    // raw buffer, 32-bit data format
 constexpr unsigned kOffNone = 0xC0000000u;      // beyond every plane (< 2^30 floats)
@@ -357,7 +365,6 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
                                          lds_vf4* tile_a, lds_vf4* tile_b, lds_vf4* tile_c) {
   constexpr bool NA = Q != 1 && Q != 4;
   constexpr bool NB = Q == 1 || Q == 2 || Q == 4;
-  constexpr int T4 = kTileStride / 4;        // tile row stride in float4
   constexpr int NE = kRowsPerWave * 8;       // float4 of one chunk: rows x 8 segments
   constexpr int NI = (NE + 63) / 64;         // coalesced instructions per chunk and plane
   const int lane = threadIdx.x;
@@ -376,9 +383,12 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
 #pragma unroll
   for (int i = 0; i < NI; ++i) {
     const int e = 64 * i + lane, er = e >> 3, es = e & 7;
-    eidx[i] = e < NE ? er * T4 + es : T4 - 1;
+    eidx[i] = e < NE ? tile_slot(er, es) : kRowsPerWave * kTileRowF4;
     voff[i] = e < NE && er < nrows ? (unsigned)(er * pitch * 4 + es * 16) : kOffNone;
   }
+  int rslot[8];  // this lane's row, segment j
+#pragma unroll
+  for (int j = 0; j < 8; ++j) rslot[j] = tile_slot(r, j);
   const int glast = (w - 1) / 4;  // last output group
   const int nchunks = glast / 8 + 1;
   float4 ring[4];
@@ -450,8 +460,8 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
     for (int j = 0; j < 8; ++j) {
       xa[j] = z4;
       xb[j] = z4;
-      if (NA) xa[j] = to_f4(tile_a[r * T4 + j]);
-      if (NB) xb[j] = to_f4(tile_b[r * T4 + j]);
+      if (NA) xa[j] = to_f4(tile_a[rslot[j]]);
+      if (NB) xb[j] = to_f4(tile_b[rslot[j]]);
     }
     __builtin_amdgcn_sched_barrier(0);
     float4 out[8];
@@ -470,7 +480,7 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
     wave_lds_order();
     if (active) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) tile_k[r * T4 + j] = to_vf4(out[j]);
+      for (int j = 0; j < 8; ++j) tile_k[rslot[j]] = to_vf4(out[j]);
     }
     wave_lds_order();
     float4 v[NI];
@@ -505,10 +515,10 @@ __global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
                                              const WorkItem* __restrict__ items) {
   // input tiles of the two images, reused with a third tile for the three
   // terms' outputs
-  __shared__ float4 tiles[3][kRowsPerWave * kTileStride / 4];
-  lds_vf4* tile_a = (lds_vf4*)tiles[0];
-  lds_vf4* tile_b = (lds_vf4*)tiles[1];
-  lds_vf4* tile_c = (lds_vf4*)tiles[2];
+  __shared__ float4 tiles[3 * kTileF4];
+  lds_vf4* tile_a = (lds_vf4*)tiles;
+  lds_vf4* tile_b = (lds_vf4*)(tiles + kTileF4);
+  lds_vf4* tile_c = (lds_vf4*)(tiles + 2 * kTileF4);
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b / kNumQ, q = it.b % kNumQ;
