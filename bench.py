@@ -160,9 +160,15 @@ def run_batch(s2, workloads, storage, host_pairs, spec, reps):
     dt_ref = (time.perf_counter() - t0) / reps
     assert np.array_equal(b.get_scores(), scores)
     b.close()
+    # the per-pair algorithmic bytes of the exact path's decomposition at this
+    # storage size (the fast form fuses more, so it can exceed 1)
+    sbytes = 4 if storage == "f32" else 2
+    alg = sum(sum(algorithmic_bytes(w, h, S=sbytes).values()) for _, _, w, h, _ in spec)
     return {
         "storage": storage,
         "pairs": len(dev),
+        "roofline": {"algorithmic_bytes": int(alg), "achieved_gbs": round(alg / dt / 1e9, 1),
+                     "frac": round(alg / dt / 1e9 / HBM_PEAK_GBS, 4)},
         "mpx_per_batch": round(px / 1e6, 3),
         "ms_per_batch": round(dt * 1e3, 4),
         "value": round(px / 1e6 / dt, 1),
