@@ -186,12 +186,14 @@ ssim2_status ssim2_has_reference(const ssim2_ctx* ctx, int* has_reference);
 /* ---- batched fast form ---------------------------------------------------- */
 /* No counterpart in the reference's ssimulacra2-cuda (its closest relative is
  * the same-size Ssim2Batch of crates/ssim2-gpu/src/pipeline_batch.rs:120-612).
- * Many sRGB pairs of unrelated sizes are scored by three persistent kernels
- * captured once in a hipGraph. Arithmetic: exact 9-tap FIR blur (the closed
- * form of the reference's recursive Gaussian), statistics in shifted
- * coordinates, f32 maps, f64 sums; XYB of scales >= 1 stored in `storage`. Its
- * scores follow the FP64 evaluation of the metric closely and
- * are not bit-identical to the exact path. */
+ * Many sRGB pairs of unrelated sizes are scored by three kernels captured
+ * once in a hipGraph (the grids that follow a batch's size are set per
+ * batch). Arithmetic: exact 9-tap FIR blur (the closed form of the
+ * reference's recursive Gaussian), statistics in shifted coordinates, f32
+ * maps, f64 sums; the XYB planes of every scale stored in `storage`. Its
+ * scores follow the FP64 evaluation of the metric closely (|dscore| about
+ * 3e-4 with f32 and 0.015 with f16 storage on natural-image-like content)
+ * and are not bit-identical to the exact path. */
 
 typedef enum {
   SSIM2_STORAGE_F32 = 0,

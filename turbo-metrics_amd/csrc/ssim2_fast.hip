@@ -1,4 +1,4 @@
-// ssimulacra2 fast form for gfx950: batched, persistent, FP32/FP16/BF16 storage.
+// ssimulacra2 fast form for gfx950: batched, FP32/FP16/BF16 storage.
 //
 // Same metric as the exact path, different arithmetic, chosen for throughput:
 //  * the blur is the 9-tap FIR that the reference's recursive Gaussian equals
@@ -17,15 +17,17 @@
 //  * the cube root is exp2(log2(x)/3) on the hardware log2/exp2, the error
 //    maps are f32 with hardware reciprocals for their two quotients, tile sums
 //    are f32 within a wave and f64 across waves and tiles;
-//  * XYB of scales >= 1 is stored in HBM as (value - shift) in the storage type
-//    (f32, IEEE half or bfloat16); scale 0 is recomputed from the u8 input
-//    inside each tile.
+//  * the XYB planes of every scale are stored in HBM as (value - shift) in
+//    the storage type (f32, IEEE half or bfloat16) by the pyramid kernel; the
+//    tile kernel reads them and never the input.
 //
-// Launches (all persistent: a fixed grid loops over a device-resident item
-// list, so one captured hipGraph serves every batch):
-//   k_fast_pyramid  32x32 scale-0 tiles of every pair: scale-1 pixels straight
-//                   from their 2x2 input blocks, linear scales >= 1 in LDS,
-//                   shifted XYB planes of scales >= 1.
+// Launches (each block takes a contiguous range of a device-resident item
+// list, so one captured hipGraph serves every batch; the grids that follow
+// the batch's size are set on the graph's nodes per batch):
+//   k_fast_pyramid  one 32x32 scale-0 tile per block: shifted XYB of scale 0
+//                   from each thread's 2x2 input block, scale-1 pixels from
+//                   the same blocks, linear scales >= 1 in LDS, shifted XYB
+//                   planes of every scale.
 //   k_fast_tile     one TXxTY output tile of one scale of one pair per item:
 //                   XYB tile + 4 px halo -> products -> horizontal FIR ->
 //                   vertical FIR -> maps -> f64 block sums.
