@@ -136,7 +136,7 @@ struct ssim2_batch {
 namespace {
 
 int* meta_ptr(int* base) { return base; }
-int* pyr_prefix(ssim2_batch* b, int* base) { return base + 4; }
+int* pyr_prefix(ssim2_batch*, int* base) { return base + 4; }
 int* tile_prefix(ssim2_batch* b, int* base) { return base + 4 + (b->max_pairs + 1); }
 
 void release(ssim2_batch* b) {

@@ -561,7 +561,7 @@ void k_fast_tile(const PairDesc* __restrict__ pairs,
       fast_barrier();
       // Horizontal FIR of the four quantities, 4 outputs per unit.
       for (int e = tid; e < IH * (TX / 4); e += kThreads) {
-        const int iy = e / (TX / 4), j0 = e % (TX / 4), ox0 = j0 * 4;
+        const int iy = e / (TX / 4), j0 = e % (TX / 4);
         float a[12], d[12];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {

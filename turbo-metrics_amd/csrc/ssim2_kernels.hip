@@ -67,10 +67,7 @@ typedef float vf4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) vf4 gfloat4;
 typedef __attribute__((address_space(1))) const vf4 gcfloat4;
 __device__ __forceinline__ float4 to_f4(vf4 v) { return make_float4(v.x, v.y, v.z, v.w); }
-__device__ __forceinline__ gcfloat* gbl(const float* p) { return (gcfloat*)p; }
 __device__ __forceinline__ gfloat* gbl(float* p) { return (gfloat*)p; }
-typedef __attribute__((address_space(1))) const unsigned char gcuchar;
-__device__ __forceinline__ gcuchar* gbl(const unsigned char* p) { return (gcuchar*)p; }
 
 // Workgroup barrier that orders LDS only. __syncthreads() also waits for every
 // outstanding global load and store (vmcnt(0)), which would drain the
@@ -80,16 +77,6 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-__device__ __forceinline__ float quantity(int q, float a, float b) {
-  switch (q) {
-    case 0: return a * a;
-    case 1: return b * b;
-    case 2: return a * b;
-    case 3: return a;
-    default: return b;
-  }
 }
 
 // XYB plane of (side image, channel) at scale s for the given launch sides.
