@@ -41,3 +41,16 @@ def test_argument_validation_without_device():
     assert L.ssim2_get_score(None, ctypes.byref(d)) == 8
     assert L.ssim2_destroy(None) == 0
     assert L.ssim2_image_alloc(0, 10, 1, None) == 8
+
+
+def test_resident_reference_argument_validation_without_device():
+    import ssimulacra2_hip as s2
+    L = s2.lib
+    rid = ctypes.c_uint32()
+    img = s2._Image()
+    assert L.ssim2_batch_add_reference(None, ctypes.byref(img), None, ctypes.byref(rid)) == 8
+    assert L.ssim2_batch_remove_reference(None, 1) == 8
+    assert L.ssim2_batch_compute_ref_async(None, None, 1, None) == 8
+    assert L.ssim2_server_add_reference(None, ctypes.byref(img), ctypes.byref(rid)) == 8
+    assert L.ssim2_server_remove_reference(None, 1) == 8
+    assert L.ssim2_server_submit_ref(None, 1, ctypes.byref(img), 0) == 8
