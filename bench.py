@@ -182,7 +182,10 @@ def run_batch(s2, workloads, storage, host_pairs, spec, reps):
     }
 
 
-def run_serving(s2, workloads, seconds, rate, storage="f16", slots=3):
+def run_serving(s2, workloads, seconds, rate, storage="f16", slots=3, resident_refs=False):
+    """The Poisson stream through the server. With resident_refs every pool
+    reference's pyramid is registered once and each request names it
+    (ssim2_server_submit_ref) instead of passing the reference image."""
     spec, cls = workloads.serving_pool_spec()
     per_class = [cls.count(c) for c in range(4)]
     host = workloads.generate(spec)
@@ -196,6 +199,7 @@ def run_serving(s2, workloads, seconds, rate, storage="f16", slots=3):
         sv.submit(*p, 10_000_000 + i)
     sv.drain(120.0)
     sv.poll()
+    ref_ids = [sv.add_reference(r) for r, _ in dev] if resident_refs else None
     results = []
     t0 = time.perf_counter()
     for i, (ta, k) in enumerate(zip(t_arr, pick)):
@@ -205,7 +209,10 @@ def run_serving(s2, workloads, seconds, rate, storage="f16", slots=3):
                 break
             if ta - now > 0.002:
                 time.sleep(0.001)
-        sv.submit(*dev[int(k)], i)
+        if ref_ids:
+            sv.submit_ref(ref_ids[int(k)], dev[int(k)][1], i)
+        else:
+            sv.submit(*dev[int(k)], i)
         if i % 256 == 0:
             results.extend(sv.poll())
     sv.drain(300.0)
@@ -537,6 +544,13 @@ def main():
             batch_legs.append(run_batch(s2, workloads, storage, bhost, bspec, args.batch_reps))
         log(f"serving {args.serving_seconds} s at {args.serving_rate} req/s")
         serving = run_serving(s2, workloads, args.serving_seconds, args.serving_rate)
+        log("serving with resident reference pyramids")
+        sref = run_serving(s2, workloads, min(args.serving_seconds, 4.0), args.serving_rate, resident_refs=True)
+        serving["resident_refs"] = {
+            "seconds": sref["seconds"], "requests": sref["requests"], "completed": sref["completed"],
+            "value": sref["value"], "unit": sref["unit"], "latency_ms": sref["latency_ms"],
+            "same_score_per_pair": sref["same_score_per_pair"],
+            "equals_direct": all(serving["pool_scores"].get(k, v) == v for k, v in sref["pool_scores"].items())}
         log("mixed-size stream: plain exact path vs batched fast form")
         pool_dev = [(s2.DeviceImage.from_host(r), s2.DeviceImage.from_host(d)) for r, d in serving["pool_host"]]
         spec, cls = workloads.serving_pool_spec()
