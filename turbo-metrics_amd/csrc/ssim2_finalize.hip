@@ -23,11 +23,6 @@ __constant__ double c_weight_fin[108] = SSIM2_WEIGHTS_INIT;
 
 constexpr int kGroups = kMaxScales * 3;
 
-__device__ __forceinline__ double wave_sum_f(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
 __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
@@ -42,8 +37,12 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const PairDesc* __rest
                                                           const int* __restrict__ meta, int npairs_static) {
   __shared__ double sums[kGroups][kNumSums];
   __shared__ double feat[kSumsPerPair];
+  __shared__ double wts[kSumsPerPair];
   const int npairs = meta ? meta[0] : npairs_static;
   const int t = threadIdx.x, lane = t & 63;
+  // the weights are read once per block, all in one round trip; the first
+  // barrier below publishes them
+  if (t < kSumsPerPair) wts[t] = c_weight_fin[t];
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   for (int pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
     const PairDesc& P = pairs[pi];
@@ -57,10 +56,18 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const PairDesc* __rest
 #pragma unroll
           for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
       }
+      // the six sums step by step together, so that each step's shuffles
+      // are in flight at once (the same tree per sum as a sum by sum)
+      for (int off = 32; off > 0; off >>= 1) {
+        double o[kNumSums];
 #pragma unroll
-      for (int q = 0; q < kNumSums; ++q) {
-        const double r = wave_sum_f(v[q]);
-        if (lane == 0) sums[g][q] = r;
+        for (int q = 0; q < kNumSums; ++q) o[q] = __shfl_down(v[q], off, 64);
+#pragma unroll
+        for (int q = 0; q < kNumSums; ++q) v[q] += o[q];
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kNumSums; ++q) sums[g][q] = v[q];
       }
     }
     lds_sync();
@@ -88,7 +95,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const PairDesc* __rest
 #pragma unroll
         for (int k = 0; k < 18; ++k) {
           fw[k] = feat[k0 + k];
-          ww[k] = c_weight_fin[k0 + k];
+          ww[k] = wts[k0 + k];
         }
 #pragma unroll
         for (int k = 0; k < 18; ++k) ssim = fma(ww[k], fw[k], ssim);
