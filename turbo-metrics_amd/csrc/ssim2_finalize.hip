@@ -52,7 +52,21 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(const PairDesc* __rest
       if (s < P.nscales) {
         const ScaleDesc& S = P.sc[s];
         const double* p = P.partials + (long long)(S.part + c * S.colblocks) * kNumSums;
-        for (int b = lane; b < S.colblocks; b += 64)
+        // records lane, lane + 64, ... in that order; four at a time so that
+        // their loads are in flight together
+        int b = lane;
+        for (; b + 192 < S.colblocks; b += 256) {
+          double r4[4][kNumSums];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int q = 0; q < kNumSums; ++q) r4[k][q] = p[(long long)(b + 64 * k) * kNumSums + q];
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int q = 0; q < kNumSums; ++q) v[q] += r4[k][q];
+        }
+        for (; b < S.colblocks; b += 64)
 #pragma unroll
           for (int q = 0; q < kNumSums; ++q) v[q] += p[(long long)b * kNumSums + q];
       }
