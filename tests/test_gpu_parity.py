@@ -40,6 +40,8 @@ CASES = [
     ("thin_9x300", 1, 10, 9, 300, 0),
     ("wide_1500x11", 2, 11, 1500, 11, 12),
     ("odd_33x65", 4, 12, 33, 65, 30),
+    # 200 column strips at scale 0: the finalisation's four-records-per-lane loop
+    ("wide_6400x16", 2, 13, 6400, 16, 8),
 ]
 
 
