@@ -171,3 +171,39 @@ def test_errors(s2):
             m.compute_srgb_sync(a, b)
         assert ei.value.status == "SSIM2_DIM_MISMATCH"
         assert m.mem_usage() > 0
+
+
+def test_calls_on_two_streams_are_ordered(s2, synth):
+    """Calls of one context on different streams, with no host sync between
+    them, score as on one stream: every call shares the context's arena,
+    partial sums and result record, so each waits for the context's previous
+    commands (the reference's calls are ordered by one graph exec,
+    crates/ssimulacra2-cuda/src/lib.rs:419-424)."""
+    w, h = 1920, 1080
+    pairs = [synth.pair(2, 61 + i, w, h, 8 + 12 * i) for i in range(3)]
+    dev = [(s2.DeviceImage.from_host(r), s2.DeviceImage.from_host(d)) for r, d in pairs]
+    a, b = s2.Stream(), s2.Stream()
+    _say("[gpu] two streams, 1920x1080")
+    with s2.Ssimulacra2(w, h) as m:
+        want = [m.compute_srgb_sync(r, d) for r, d in dev]
+        assert len(set(want)) == 3
+        # async on A, then async on B: B's kernels must not overlap A's
+        m.compute_srgb(*dev[0], stream=a)
+        m.compute_srgb(*dev[1], stream=b)
+        assert m.get_score() == want[1]
+        for k in range(6):
+            m.compute_srgb(*dev[k % 3], stream=a if k % 2 else b)
+        assert m.get_score() == want[2]
+        # a reference set on B while A's compute may still run: get_score
+        # afterwards is A's result, and scoring against the reference on A
+        # gives the direct score
+        m.compute_srgb(*dev[0], stream=a)
+        m.set_reference_srgb(dev[2][0], stream=b)
+        assert m.get_score() == want[0]
+        assert m.compute_with_reference_srgb(dev[2][1], stream=a) == want[2]
+        # host-byte uploads into the shared staging images on alternating streams
+        for k in range(4):
+            r, d = pairs[k % 3]
+            assert m.compute_from_cpu_srgb_sync(r, d, stream=a if k % 2 else b) == want[k % 3]
+    a.close()
+    b.close()

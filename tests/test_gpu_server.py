@@ -116,3 +116,47 @@ def test_server_tight_budget_serialises(s2, pool):
         st = sv.stats()
     assert [r[1] for r in res] == [want] * 6
     assert st["batches"] >= 3
+
+
+def test_server_remove_reference_while_queued(s2, pool):
+    """A resident reference removed while requests naming it wait: those
+    requests fail alone with SSIM2_NO_REFERENCE (or score normally if they ran
+    before the removal), and every other request of their batches scores as
+    it does alone."""
+    with s2.Ssim2Batch(max_pairs=8, max_pixels=2_000_000, storage="f16") as b:
+        alone = [b.compute([p])[0] for p in pool]
+    _say("[gpu] server: remove a reference with 150 of its requests queued")
+    with s2.Ssim2Server(storage="f16", slots=1, max_pairs_per_batch=2, max_pixels_per_batch=4_000_000,
+                        hbm_budget=1 << 30) as sv:
+        ref_id = sv.add_reference(pool[4][0])
+        kinds = {}
+        for rid in range(300):
+            if rid % 2 == 0:
+                sv.submit_ref(ref_id, pool[4][1], rid)
+                kinds[rid] = None
+            else:
+                k = rid % len(pool)
+                sv.submit(*pool[k], rid)
+                kinds[rid] = k
+        sv.remove_reference(ref_id)
+        for rid in range(300, 320):
+            k = rid % len(pool)
+            sv.submit(*pool[k], rid)
+            kinds[rid] = k
+        sv.drain(60.0)
+        res = sv.poll()
+    assert len(res) == len(kinds)
+    stale = 0
+    for rid, score, _, _, status in res:
+        k = kinds[rid]
+        if k is None:
+            assert status in ("SSIM2_OK", "SSIM2_NO_REFERENCE"), (rid, status)
+            if status == "SSIM2_OK":
+                assert score == alone[4], (rid, score, alone[4])
+            else:
+                stale += 1
+        else:
+            assert status == "SSIM2_OK", (rid, status)
+            assert score == alone[k], (rid, score, alone[k])
+    _say(f"[gpu] {stale} of 150 reference requests were still queued at removal")
+    assert stale >= 1

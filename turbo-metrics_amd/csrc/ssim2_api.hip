@@ -55,6 +55,11 @@ struct ssim2_ctx {
   size_t bytes = 0;
   bool has_ref = false;
   bool have_result = false;
+  // Every call shares the arena, partial sums, staging images and result
+  // record, whichever stream it names: `done` is recorded after each call's
+  // last command, and a call on another stream than `last` first waits for it.
+  hipEvent_t done = nullptr;
+  bool issued = false;
   hipStream_t last = nullptr;
 };
 
@@ -68,6 +73,8 @@ void release(ssim2_ctx* c) {
   if (c->d_desc) (void)hipFree(c->d_desc);
   if (c->d_items) (void)hipFree(c->d_items);
   if (c->stage) (void)hipFree(c->stage);
+  if (c->done) (void)hipEventDestroy(c->done);
+  c->done = nullptr;
   c->arena = nullptr;
   c->partials = c->result = c->result_host = nullptr;
   c->d_desc = nullptr;
@@ -134,9 +141,27 @@ InputArgs make_inputs(const ssim2_image* ref, const ssim2_image* dis, int kind) 
   return in;
 }
 
+// Orders stream `s` behind every command the context has issued on other
+// streams (the reference's calls are ordered the same way: one graph exec,
+// crates/ssimulacra2-cuda/src/lib.rs:419-424).
+ssim2_status join_stream(ssim2_ctx* c, hipStream_t s) {
+  if (c->issued && c->last != s) HIP_TRY(hipStreamWaitEvent(s, c->done, 0));
+  c->last = s;
+  return SSIM2_OK;
+}
+
+// Marks the end of the commands just issued on `s`.
+ssim2_status mark_issued(ssim2_ctx* c, hipStream_t s) {
+  HIP_TRY(hipEventRecord(c->done, s));
+  c->issued = true;
+  return SSIM2_OK;
+}
+
 // Enqueue the pipeline for the context's single pair.
 ssim2_status enqueue(ssim2_ctx* c, int sides, const InputArgs& in, hipStream_t s) {
   const PairPlan& p = c->plan;
+  ssim2_status st = join_stream(c, s);
+  if (st != SSIM2_OK) return st;
   HIP_TRY(launch_pyramid(c->d_desc, c->d_items + p.off_pyr, p.n_pyr, sides, in, s));
   HIP_TRY(launch_rows(c->d_desc, c->d_items + p.off_rows[sides], p.n_rows[sides], sides, s));
   HIP_TRY(launch_cols(c->d_desc, c->d_items + p.off_cols, p.n_cols, sides, s));
@@ -146,13 +171,14 @@ ssim2_status enqueue(ssim2_ctx* c, int sides, const InputArgs& in, hipStream_t s
                            hipMemcpyDeviceToHost, s));
     c->have_result = true;
   }
-  c->last = s;
-  return SSIM2_OK;
+  return mark_issued(c, s);
 }
 
+// Every command is ordered behind the previous one, so the last recorded
+// event covers the last result copy whatever stream it ran on.
 ssim2_status wait_result(ssim2_ctx* c) {
   if (!c->have_result) return fail(SSIM2_INVALID_ARGUMENT, "no computation has been enqueued");
-  HIP_TRY(hipStreamSynchronize(c->last));
+  HIP_TRY(hipEventSynchronize(c->done));
   return SSIM2_OK;
 }
 
@@ -220,7 +246,8 @@ ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* 
       (e = hipHostMalloc(&c->result_host, kResultDoubles * sizeof(double), hipHostMallocDefault)) !=
           hipSuccess ||
       (e = hipMalloc(&c->d_desc, sizeof(PairDesc))) != hipSuccess ||
-      (e = hipMalloc(&c->d_items, items_bytes)) != hipSuccess) {
+      (e = hipMalloc(&c->d_items, items_bytes)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming)) != hipSuccess) {
     release(c);
     delete c;
     (void)hipGetLastError();
@@ -247,8 +274,8 @@ ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* 
 
 ssim2_status ssim2_destroy(ssim2_ctx* c) {
   if (!c) return SSIM2_OK;
-  if (c->last || c->have_result) (void)hipStreamSynchronize(c->last);
-  (void)hipDeviceSynchronize();
+  // only this context's own commands are waited for (Drop, lib.rs:1195-1202)
+  if (c->issued) (void)hipEventSynchronize(c->done);
   release(c);
   delete c;
   return SSIM2_OK;
@@ -316,7 +343,7 @@ ssim2_status ssim2_compute_srgb_sync(ssim2_ctx* c, const ssim2_image* ref, const
   return ssim2_get_score(c, score);
 }
 
-ssim2_status ensure_stage(ssim2_ctx* c) {
+static ssim2_status ensure_stage(ssim2_ctx* c) {
   if (c->stage) return SSIM2_OK;
   c->stage_pitch = (size_t(c->w) * 3 + 255) / 256 * 256;
   hipError_t e = hipMalloc(&c->stage, 2 * c->stage_pitch * size_t(c->h));
@@ -329,16 +356,17 @@ ssim2_status ensure_stage(ssim2_ctx* c) {
 }
 
 // Uploads `host` into staging image `which` (0 ref, 1 dis) on `s`.
-ssim2_status upload_stage(ssim2_ctx* c, int which, const uint8_t* host, size_t row_bytes, hipStream_t s,
+static ssim2_status upload_stage(ssim2_ctx* c, int which, const uint8_t* host, size_t row_bytes, hipStream_t s,
                           ssim2_image* out) {
   const size_t rb = size_t(c->w) * 3;
   if (row_bytes < rb) return fail(SSIM2_INVALID_ARGUMENT, "row_bytes smaller than 3*width");
   ssim2_status st = ensure_stage(c);
   if (st != SSIM2_OK) return st;
   unsigned char* p = c->stage + (which ? c->stage_pitch * size_t(c->h) : 0);
+  if ((st = join_stream(c, s)) != SSIM2_OK) return st;
   HIP_TRY(hipMemcpy2DAsync(p, c->stage_pitch, host, row_bytes, rb, c->h, hipMemcpyHostToDevice, s));
   *out = ssim2_image{p, c->stage_pitch, (uint32_t)c->w, (uint32_t)c->h};
-  return SSIM2_OK;
+  return mark_issued(c, s);
 }
 
 ssim2_status ssim2_compute_from_host_srgb(ssim2_ctx* c, const uint8_t* ref, const uint8_t* dis,
@@ -479,6 +507,7 @@ ssim2_status ssim2_time_kernels(ssim2_ctx* c, const ssim2_image* ref, const ssim
   hipStream_t s = (hipStream_t)stream;
   const PairPlan& p = c->plan;
   const InputArgs in = make_inputs(ref, dis, kInputSrgbU8);
+  if ((st = join_stream(c, s)) != SSIM2_OK) return st;
   hipEvent_t ev[5];
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   double acc[4] = {0, 0, 0, 0};
@@ -510,8 +539,7 @@ ssim2_status ssim2_time_kernels(ssim2_ctx* c, const ssim2_image* ref, const ssim
   if (out != SSIM2_OK) return out;
   for (int k = 0; k < 4; ++k) ms[k] = (float)(acc[k] / reps);
   c->have_result = true;
-  c->last = s;
-  return SSIM2_OK;
+  return mark_issued(c, s);
 }
 
 ssim2_status ssim2_device_copy(void* dst, const void* src, size_t bytes, ssim2_stream stream) {

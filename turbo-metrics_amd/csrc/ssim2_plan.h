@@ -47,12 +47,77 @@ struct PairPlan {
   int off_rows[3] = {0, 0, 0}, n_rows[3] = {0, 0, 0};  // indexed by Sides
 };
 
-// Lay out one pair's planes starting at element `arena_base` and its partial
-// records starting at `part_base`; `pair` is written into every work item.
-inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPlan* out, int pair = 0) {
-  PairPlan& p = *out;
-  p = PairPlan{};
-  PairDesc& d = p.desc;
+// Work lists. A pair's items are the pyramid items, then the row items of
+// each Sides value, then the column items; item k of a section is a function
+// of the pair's scale sizes alone, so the host planner and a device-side
+// planner (the no-allocation entry point) produce the same list.
+
+// Pyramid: 32x32 scale-0 tiles, row-major.
+__host__ __device__ inline int pyramid_items(const PairDesc& d) {
+  return ((d.w + kPyrTile - 1) / kPyrTile) * ((d.h + kPyrTile - 1) / kPyrTile);
+}
+__host__ __device__ inline WorkItem pyramid_item(const PairDesc& d, int k, int pair) {
+  const int tx = (d.w + kPyrTile - 1) / kPyrTile;
+  return {pair, k % tx, k / tx, 0};
+}
+
+// Row pass: one wave per (scale, channel*5+quantity, band of kRowsPerWave
+// rows), largest scale first; each Sides value runs its quantities only.
+// The waves of one band read the same two XYB rows, so they are placed on
+// one XCD: workgroups go to the 8 XCDs round-robin by id, and in each group
+// of 8 bands the quantity waves of band j get ids = j (mod 8), so they share
+// that XCD's L2.
+__host__ __device__ inline int row_quantities(int sides) { return sides == kSidesBoth ? 5 : (sides == kSidesRef ? 2 : 3); }
+// Quantity index (ref^2, dis^2, ref*dis, ref, dis) of the qi-th quantity a Sides value runs.
+__host__ __device__ inline int row_quantity_id(int sides, int qi) {
+  if (sides == kSidesBoth) return qi;
+  if (sides == kSidesRef) return qi ? 3 : 0;
+  return qi == 0 ? 1 : (qi == 1 ? 2 : 4);
+}
+__host__ __device__ inline int row_bands(const PairDesc& d) {
+  int n = 0;
+  for (int s = 0; s < d.nscales; ++s) n += 3 * ((d.sc[s].h + kRowsPerWave - 1) / kRowsPerWave);
+  return n;
+}
+__host__ __device__ inline int row_items(const PairDesc& d, int sides) { return row_quantities(sides) * row_bands(d); }
+// Bands are numbered scale by scale, channel by channel, band by band.
+__host__ __device__ inline WorkItem row_item(const PairDesc& d, int sides, int k, int pair) {
+  const int nq = row_quantities(sides), total = row_bands(d);
+  const int g = k / (8 * nq);             // group of 8 bands (the last may hold fewer)
+  const int n = total - 8 * g < 8 ? total - 8 * g : 8;
+  const int t = k - 8 * nq * g, qi = t / n;
+  int band = 8 * g + t % n;
+  for (int s = 0; s < d.nscales; ++s) {
+    const int nb = (d.sc[s].h + kRowsPerWave - 1) / kRowsPerWave;
+    if (band < 3 * nb) return {pair, s, (band / nb) * kNumQ + row_quantity_id(sides, qi), band % nb};
+    band -= 3 * nb;
+  }
+  return {pair, 0, 0, 0};
+}
+
+// Column pass + maps: (scale, channel, kColBlock-column strip).
+__host__ __device__ inline int column_items(const PairDesc& d) {
+  int n = 0;
+  for (int s = 0; s < d.nscales; ++s) n += 3 * d.sc[s].colblocks;
+  return n;
+}
+__host__ __device__ inline WorkItem column_item(const PairDesc& d, int k, int pair) {
+  for (int s = 0; s < d.nscales; ++s) {
+    const int nb = d.sc[s].colblocks;
+    if (k < 3 * nb) return {pair, s, k / nb, k % nb};
+    k -= 3 * nb;
+  }
+  return {pair, 0, 0, 0};
+}
+
+// Scale sizes, pitches and plane offsets of one pair whose planes start at
+// element `arena_base` and whose partial records start at `part_base`;
+// `planes` planes per scale (kPlanesPerScale, or 6 + 3 * kNumQ without the
+// cached-reference planes). Returns the arena elements used.
+inline long long layout_pair(int w, int h, long long arena_base, int part_base, int planes, PairDesc* out,
+                             int* nparts) {
+  PairDesc& d = *out;
+  d = PairDesc{};
   d.w = w;
   d.h = h;
   int sw[kMaxScales], sh[kMaxScales];
@@ -73,48 +138,32 @@ inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPla
     S.xyb = off;
     S.hrow = off + 6 * S.plane;
     S.cache = off + (6 + 3 * kNumQ) * S.plane;
-    off += kPlanesPerScale * S.plane;
+    off += planes * S.plane;
     S.part = part;
     part += 3 * S.colblocks;
   }
-  p.arena_elems = off - arena_base;
-  p.nparts = part - part_base;
-  // Pyramid: 32x32 scale-0 tiles.
+  *nparts = part - part_base;
+  return off - arena_base;
+}
+
+// Lay out one pair's planes starting at element `arena_base` and its partial
+// records starting at `part_base`; `pair` is written into every work item.
+inline void plan_pair(int w, int h, long long arena_base, int part_base, PairPlan* out, int pair = 0) {
+  PairPlan& p = *out;
+  p = PairPlan{};
+  p.arena_elems = layout_pair(w, h, arena_base, part_base, kPlanesPerScale, &p.desc, &p.nparts);
+  const PairDesc& d = p.desc;
   p.off_pyr = 0;
-  for (int ty = 0; ty < (h + kPyrTile - 1) / kPyrTile; ++ty)
-    for (int tx = 0; tx < (w + kPyrTile - 1) / kPyrTile; ++tx) p.items.push_back({pair, tx, ty, 0});
-  p.n_pyr = (int)p.items.size();
-  // Row pass: one wave per (scale, channel*5+quantity, band of kRowsPerWave
-  // rows), largest scale first; one list per Sides (its quantities only).
-  // The waves of one band read the same two XYB rows, so they are placed on
-  // one XCD: workgroups go to the 8 XCDs round-robin by id, and in each group
-  // of 8 bands the quantity waves of band j get ids = j (mod 8), so they
-  // share that XCD's L2.
-  static const int kQ[3][5] = {{0, 1, 2, 3, 4}, {0, 3, -1, -1, -1}, {1, 2, 4, -1, -1}};
+  p.n_pyr = pyramid_items(d);
+  for (int k = 0; k < p.n_pyr; ++k) p.items.push_back(pyramid_item(d, k, pair));
   for (int sd = 0; sd < 3; ++sd) {
     p.off_rows[sd] = (int)p.items.size();
-    int nq = 0;
-    while (nq < 5 && kQ[sd][nq] >= 0) ++nq;
-    std::vector<WorkItem> bands;  // (scale, channel, band), largest scale first
-    for (int s = 0; s < d.nscales; ++s)
-      for (int c = 0; c < 3; ++c)
-        for (int b = 0; b < (sh[s] + kRowsPerWave - 1) / kRowsPerWave; ++b) bands.push_back({pair, s, c, b});
-    for (size_t g = 0; g < bands.size(); g += 8) {
-      const size_t n = std::min<size_t>(8, bands.size() - g);
-      for (int qi = 0; qi < nq; ++qi)
-        for (size_t j = 0; j < n; ++j) {
-          const WorkItem& B = bands[g + j];
-          p.items.push_back({pair, B.a, B.b * kNumQ + kQ[sd][qi], B.c});
-        }
-    }
-    p.n_rows[sd] = (int)p.items.size() - p.off_rows[sd];
+    p.n_rows[sd] = row_items(d, sd);
+    for (int k = 0; k < p.n_rows[sd]; ++k) p.items.push_back(row_item(d, sd, k, pair));
   }
-  // Column pass + maps: (scale, channel, 16-column strip).
   p.off_cols = (int)p.items.size();
-  for (int s = 0; s < d.nscales; ++s)
-    for (int c = 0; c < 3; ++c)
-      for (int b = 0; b < d.sc[s].colblocks; ++b) p.items.push_back({pair, s, c, b});
-  p.n_cols = (int)p.items.size() - p.off_cols;
+  p.n_cols = column_items(d);
+  for (int k = 0; k < p.n_cols; ++k) p.items.push_back(column_item(d, k, pair));
 }
 
 }  // namespace ssim2

@@ -104,6 +104,12 @@ struct ssim2_server {
   };
   std::vector<RefInfo> refs;
 
+  // Whether `id` names a registered reference; call with ref_mu held.
+  bool has_ref(uint32_t id) const {
+    for (const RefInfo& r : refs)
+      if (r.id == id) return true;
+    return false;
+  }
   void run();
 };
 
@@ -165,19 +171,51 @@ void ssim2_server::run() {
         s.bytes = bytes;
         in_flight_bytes += bytes;
       }
-      s.pairs.resize(s.reqs.size());
-      s.ref_ids.resize(s.reqs.size());
-      for (size_t i = 0; i < s.reqs.size(); ++i) {
-        s.pairs[i] = s.reqs[i].pair;
-        s.ref_ids[i] = s.reqs[i].ref_id;
-      }
       s.t_launch = now_s();
-      ssim2_status st;
+      ssim2_status st = SSIM2_OK;
+      std::vector<Request> stale;
       {
         std::lock_guard<std::mutex> rk(ref_mu);
-        st = batch_compute_mixed_async(s.batch, s.pairs.data(), s.ref_ids.data(), (uint32_t)s.pairs.size(),
-                                       s.stream);
+        // A resident reference removed after its requests were queued: those
+        // requests fail on their own, the rest of the batch runs as it would
+        // have without them. Holding ref_mu from this check to the launch
+        // keeps the reference from being removed in between.
+        size_t k = 0;
+        for (size_t i = 0; i < s.reqs.size(); ++i) {
+          if (s.reqs[i].ref_id && !has_ref(s.reqs[i].ref_id)) {
+            stale.push_back(s.reqs[i]);
+            continue;
+          }
+          s.reqs[k++] = s.reqs[i];
+        }
+        s.reqs.resize(k);
+        s.pairs.resize(k);
+        s.ref_ids.resize(k);
+        for (size_t i = 0; i < k; ++i) {
+          s.pairs[i] = s.reqs[i].pair;
+          s.ref_ids[i] = s.reqs[i].ref_id;
+        }
+        if (k)
+          st = batch_compute_mixed_async(s.batch, s.pairs.data(), s.ref_ids.data(), (uint32_t)k, s.stream);
       }
+      if (!stale.empty()) {
+        std::lock_guard<std::mutex> lk(mu);
+        const double t = now_s();
+        for (const Request& q : stale) {
+          ssim2_result r{};
+          r.request_id = q.id;
+          r.status = SSIM2_NO_REFERENCE;
+          r.submit_time_s = q.t_submit;
+          r.done_time_s = t;
+          done.push_back(r);
+          s.bytes -= q.bytes;
+          in_flight_bytes -= q.bytes;
+        }
+        n_done += stale.size();
+        progressed = true;
+        cv.notify_all();
+      }
+      if (s.reqs.empty()) continue;
       if (st != SSIM2_OK) {
         std::lock_guard<std::mutex> lk(mu);
         worker_error = ssim2_last_error();
