@@ -127,6 +127,25 @@ ssim2_status ssim2_compute_srgb_sync(ssim2_ctx* ctx, const ssim2_image* ref_srgb
 ssim2_status ssim2_compute_srgb_async(ssim2_ctx* ctx, const ssim2_image* ref_srgb,
                                       const ssim2_image* dis_srgb, ssim2_stream stream);
 
+/* Doubles in a result record: [0] score, [1..108] the per-scale sums
+ * (scale*18 + channel*6 + {ssim S1, ssim S4, artifact S1, artifact S4,
+ * detail-loss S1, detail-loss S4}), [109..120] the scale widths and heights. */
+#define SSIM2_RESULT_DOUBLES 121
+
+/* No counterpart in the reference. ssim2_compute_srgb_async for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): no context, no allocation, no host
+ * copy. The same kernels run on `stream` and the score equals the context
+ * path's to the bit. `result` is a device buffer of SSIM2_RESULT_DOUBLES
+ * doubles that receives the record when the stream reaches it.
+ * The working memory is one device array per GPU, held until the process
+ * ends; it fits pairs of up to about 2.2 Mpx (a 1920x1080 pair needs
+ * 244 MB of the 256 MiB): larger pairs are refused with SSIM2_INVALID_SIZE.
+ * Calls that overlap on one GPU share that array and race: the caller orders
+ * them (one stream, or events). */
+ssim2_status ssim2_compute_srgb_noalloc_async(const ssim2_image* ref_srgb, const ssim2_image* dis_srgb,
+                                              double* result, ssim2_stream stream);
+
 /* Ssimulacra2::compute_from_cpu_srgb_sync (lib.rs:329-345): packed sRGB bytes
  * in host memory (`row_bytes` apart, at least 3*width), uploaded into the
  * context's own staging images. */

@@ -195,6 +195,7 @@ def _load():
         "ssim2_compute_linear_sync": (vp, img, img, vp, P(dbl)),
         "ssim2_compute_srgb_sync": (vp, img, img, img, img, vp, P(dbl)),
         "ssim2_compute_srgb_async": (vp, img, img, vp),
+        "ssim2_compute_srgb_noalloc_async": (img, img, vp, vp),
         "ssim2_compute_from_host_srgb": (vp, vp, vp, sz, vp, P(dbl)),
         "ssim2_prepare_reference_from_host_srgb": (vp, vp, sz, vp),
         "ssim2_compute_from_host_dis_srgb": (vp, vp, sz, vp, P(dbl)),
@@ -437,6 +438,14 @@ class DeviceBuffer:
     def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None, stream=None):
         _check(lib.ssim2_device_copy(self.ptr, other.ptr, nbytes or min(self.nbytes, other.nbytes),
                                      _stream_ptr(stream)))
+
+    def download(self, dtype=np.uint8) -> np.ndarray:
+        """Synchronous copy of the whole buffer to the host (as one image row
+        of `nbytes` bytes)."""
+        out = np.empty(self.nbytes, np.uint8)
+        row = _Image(self.ptr, self.nbytes, 1, 1)
+        _check(lib.ssim2_image_download(ctypes.byref(row), out.ctypes.data, self.nbytes))
+        return out.view(dtype)
 
     def free(self):
         if self.ptr:
