@@ -45,6 +45,16 @@
 //     cpu.rs:931-948; the double IIR equals the 9-tap FIR used by MODE_F64.
 //   * XYB: closed form of cpu.rs:458-496, compared with an independent double
 //     evaluation; downscale against brute force on odd sizes.
+//   * error maps: hand-worked pixels with closed-form values (equal means:
+//     d from the variances only; src = mu1: pure artifact; dis = mu2: pure
+//     detail loss), for the mirror's and the fast form's maps.
+//   * score: each of the 108 features alone against cpu.rs:840-868's
+//     transform with the fixture's weights, over six and over four scales.
+//   * fast modes: round_f16 equal to IEEE binary16 (numpy) on every float
+//     below 65520 of either sign; MODE_FAST_SHIFT within 1e-3 of MODE_F64 on
+//     tiny and odd images (the shifted-statistics identities are exact);
+//     MODE_FAST_SHIFT_F16_ALL within 0.05 of both. Each of these pins fails
+//     with the matching defect planted (oracle_plant).
 //   * whole pipeline: identical inputs score exactly 100 (cpu.rs:862-868),
 //     JPEG corpus scores of the published ssimulacra2 0.5.1 crate (loose),
 //     monotone in distortion strength, FP32 mirror vs FP64 within bounds.
@@ -63,6 +73,15 @@
 #include "ref_tables.h"
 
 namespace {
+
+// Planted defects (oracle_plant): 0 everywhere except in the CPU tests that
+// show each pin of the oracle fails on a plausible mistake.
+//   1  shifted statistics without the zero-padding corrections (e = 0)
+//   2  binary16 rounding by truncation instead of to nearest even
+//   3  edge maps on the signed difference (the absolute value dropped)
+//   4  SSIM map without the luminance term num_m
+//   5  score weights indexed as if all six scales were present
+int g_plant = 0;
 
 const float kLut[256] = ORACLE_SRGB8_LUT_INIT;
 const double kWeight[108] = ORACLE_WEIGHTS_INIT;
@@ -216,7 +235,7 @@ void ssim_sums_f32(int n, const float* m1, const float* m2, const float* s11, co
     float mu1 = m1[i], mu2 = m2[i];
     float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
     float mu_diff = mu1 - mu2;
-    float num_m = std::fma(mu_diff, -mu_diff, 1.0f);
+    float num_m = g_plant == 4 ? 1.0f : std::fma(mu_diff, -mu_diff, 1.0f);
     float num_s = std::fma(2.0f, s12[i] - mu12, C2);
     float denom_s = ((s11[i] - mu11) + (s22[i] - mu22)) + C2;
     double d = 1.0 - double((num_m * num_s) / denom_s);
@@ -234,7 +253,9 @@ void edge_sums_f32(int n, const float* i1, const float* m1, const float* i2, con
                    double* out4) {
   double s[4] = {0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
-    double d1 = (1.0 + double(std::fabs(i2[i] - m2[i]))) / (1.0 + double(std::fabs(i1[i] - m1[i]))) - 1.0;
+    const float e2d = i2[i] - m2[i], e1d = i1[i] - m1[i];
+    double d1 = g_plant == 3 ? (1.0 + double(e2d)) / (1.0 + double(e1d)) - 1.0
+                             : (1.0 + double(std::fabs(e2d))) / (1.0 + double(std::fabs(e1d))) - 1.0;
     double art = std::max(d1, 0.0);
     double a2 = art * art;
     s[0] += art;
@@ -259,11 +280,11 @@ inline float round_f16(float f) {
     float a;
     std::memcpy(&a, &ax, 4);
     const float q = 0x1.0p-24f;
-    const float r = std::nearbyint(a / q) * q;
+    const float r = (g_plant == 2 ? std::trunc(a / q) : std::nearbyint(a / q)) * q;
     return sign ? -r : r;
   }
   const uint32_t lsb = (ax >> 13) & 1u;
-  ax += 0xfffu + lsb;
+  if (g_plant != 2) ax += 0xfffu + lsb;
   ax &= ~0x1fffu;
   const uint32_t y = sign | ax;
   float r;
@@ -296,14 +317,16 @@ void map_sums_fast(int n, const float* src, const float* dis, const float* m1, c
   double a[6] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < n; ++i) {
     float md = m1[i] - m2[i];
-    float num_m = std::fma(md, -md, 1.0f);
+    float num_m = g_plant == 4 ? 1.0f : std::fma(md, -md, 1.0f);
     float num_s = std::fma(2.0f, s12[i] - m1[i] * m2[i], C2);
     float den = ((s11[i] - m1[i] * m1[i]) + (s22[i] - m2[i] * m2[i])) + C2;
     float d = std::max(1.0f - (num_m * num_s) / den, 0.0f);
     float d2 = d * d;
     a[0] += d;
     a[1] += d2 * d2;
-    float d1 = (1.0f + std::fabs(dis[i] - m2[i])) / (1.0f + std::fabs(src[i] - m1[i])) - 1.0f;
+    const float e2d = dis[i] - m2[i], e1d = src[i] - m1[i];
+    float d1 = g_plant == 3 ? (1.0f + e2d) / (1.0f + e1d) - 1.0f
+                            : (1.0f + std::fabs(e2d)) / (1.0f + std::fabs(e1d)) - 1.0f;
     float art = std::max(d1, 0.0f), det = std::max(-d1, 0.0f);
     float a2 = art * art, e2 = det * det;
     a[2] += art;
@@ -337,7 +360,7 @@ void shifted_scale_sums(const std::vector<float>* p1, const std::vector<float>* 
     blur_fir<float>(q.data(), s12.data(), w, h, tmp);
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
-      const float e = 1.0f - b1[i];
+      const float e = g_plant == 1 ? 0.0f : 1.0f - b1[i];
       const float cb = sh * b1[i];
       const float var1 = (s11[i] - m1[i] * m1[i]) + e * sh * (2.0f * m1[i] + cb);
       const float var2 = (s22[i] - m2[i] * m2[i]) + e * sh * (2.0f * m2[i] + cb);
@@ -684,6 +707,21 @@ void oracle_map_sums_f32(int n, const float* src, const float* dis, const float*
   edge_sums_f32(n, src, mu1, dis, mu2, out6 + 2);
 }
 
+// Fast-form f32 error-map sums for one channel (the arithmetic of modes 3-6);
+// out6 as oracle_map_sums_f32.
+void oracle_map_sums_fast(int n, const float* src, const float* dis, const float* mu1, const float* mu2,
+                          const float* s11, const float* s22, const float* s12, double* out6) {
+  map_sums_fast(n, src, dis, mu1, mu2, s11, s22, s12, out6);
+}
+
+// The binary16 round trip the FP16-storage modes apply, element-wise.
+void oracle_round_f16(const float* in, size_t n, float* out) {
+  for (size_t i = 0; i < n; ++i) out[i] = round_f16(in[i]);
+}
+
+// Selects a planted defect (see g_plant); 0 restores the oracle.
+void oracle_plant(int which) { g_plant = which; }
+
 // Scale count and sizes per cpu.rs:358-368 (the first sub-8 scale is scored).
 int oracle_scale_dims(int w, int h, int* dims) {
   int n = 0;
@@ -706,6 +744,7 @@ double oracle_score_from_sums(const double* sums, const int* dims, int nscales) 
   int i = 0;
   for (int c = 0; c < 3; ++c)
     for (int s = 0; s < nscales; ++s) {
+      if (g_plant == 5) i = c * 36 + s * 6;
       const double opp = 1.0 / double(size_t(dims[2 * s]) * size_t(dims[2 * s + 1]));
       const double* q = sums + size_t(s) * 18 + c * 6;
       const double ssim_avg[2] = {opp * q[0], std::sqrt(std::sqrt(opp * q[1]))};

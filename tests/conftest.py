@@ -1,5 +1,6 @@
 """Shared fixtures: the package path, the CPU oracle (test infrastructure) and
 the synthetic generators."""
+import contextlib
 import ctypes
 import sys
 from pathlib import Path
@@ -45,6 +46,9 @@ class Oracle:
                   "oracle_blur_f64"):
             getattr(L, n).argtypes = [vp, i, i, vp]
         L.oracle_map_sums_f32.argtypes = [i] + [vp] * 7 + [vp]
+        L.oracle_map_sums_fast.argtypes = [i] + [vp] * 7 + [vp]
+        L.oracle_round_f16.argtypes = [vp, ctypes.c_size_t, vp]
+        L.oracle_plant.argtypes = [i]
         L.oracle_score_srgb_many.argtypes = [vp, vp, vp, vp, i, i, i, vp]
         L.oracle_bt709_coeffs.argtypes = [vp]
         L.oracle_bt709_eotf.argtypes = [d]
@@ -147,6 +151,30 @@ class Oracle:
         s = ctypes.c_double()
         self.L.oracle_score_yuv420p10_f64(*[p.ctypes.data for p in planes], w, h, ctypes.byref(s))
         return s.value
+
+    def map_sums(self, src, dis, mu1, mu2, s11, s22, s12, fast=False):
+        """Error-map sums {ssim S1, S4, art S1, S4, det S1, S4} of one channel:
+        the mirror's (cpu.rs:581-683) or, with fast=True, the fast form's f32 maps."""
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (src, dis, mu1, mu2, s11, s22, s12)]
+        out = np.zeros(6)
+        fn = self.L.oracle_map_sums_fast if fast else self.L.oracle_map_sums_f32
+        fn(arrs[0].size, *[a.ctypes.data for a in arrs], out.ctypes.data)
+        return out
+
+    def round_f16(self, x):
+        a = np.ascontiguousarray(x, np.float32)
+        out = np.empty_like(a)
+        self.L.oracle_round_f16(a.ctypes.data, a.size, out.ctypes.data)
+        return out
+
+    @contextlib.contextmanager
+    def planted(self, which):
+        """Run the oracle with planted defect `which` (see ssim2_oracle.cpp g_plant)."""
+        self.L.oracle_plant(which)
+        try:
+            yield
+        finally:
+            self.L.oracle_plant(0)
 
     def scale_dims(self, w, h):
         d = np.zeros(12, np.int32)

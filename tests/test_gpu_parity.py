@@ -207,3 +207,40 @@ def test_calls_on_two_streams_are_ordered(s2, synth):
             assert m.compute_from_cpu_srgb_sync(r, d, stream=a if k % 2 else b) == want[k % 3]
     a.close()
     b.close()
+
+
+def test_jpeg_corpus_through_gpu_paths(s2, oracle):
+    """The reference-held JPEG corpus (crates/dssim-cuda/test_data) through the
+    exact and the batched paths, against the published CPU ssimulacra2 0.5.1
+    scores (ssim2-gpu/PORT_STATUS.md:28-33). That crate's sRGB transfer and
+    cube root differ from cpu.rs and PIL decodes the JPEGs here, so the check
+    against the published values is loose (0.25, as for the oracle); the exact
+    path equals the oracle's f32 mirror and the batched forms are within 0.05
+    of FP64 on the same decoded bytes."""
+    from conftest import GOLDEN
+    from PIL import Image
+
+    corpus = GOLDEN / "jpeg_corpus"
+    src = np.asarray(Image.open(corpus / "source.png").convert("RGB"))
+    h, w = src.shape[:2]
+    published = {1: 1.2391, 5: -10.4452, 20: 57.0726, 45: 68.6823, 70: 79.5139, 90: 90.8900}
+    dis = {q: np.asarray(Image.open(corpus / f"q{q}.jpg").convert("RGB")) for q in published}
+    R = s2.DeviceImage.from_host(src)
+    D = {q: s2.DeviceImage.from_host(d) for q, d in dis.items()}
+    _say(f"[gpu] JPEG corpus {w}x{h}, {len(published)} qualities")
+    with s2.Ssimulacra2(w, h) as m:
+        exact = {q: m.compute_srgb_sync(R, D[q]) for q in published}
+    fast = {}
+    for storage in ("f32", "f16"):
+        with s2.Ssim2Batch(max_pairs=len(published), max_pixels=len(published) * w * h, storage=storage) as b:
+            fast[storage] = dict(zip(published, b.compute([(R, D[q]) for q in published])))
+    for q, want in published.items():
+        mirror = oracle.score_srgb(src, dis[q])
+        f64 = oracle.score_srgb(src, dis[q], 1)
+        _say(f"[gpu] q{q}: exact {exact[q]:.4f} f32 {fast['f32'][q]:.4f} f16 {fast['f16'][q]:.4f} "
+             f"f64 {f64:.4f} published {want:.4f}")
+        assert abs(exact[q] - mirror) <= SCORE_TOL
+        assert abs(exact[q] - want) < 0.25, (q, exact[q], want)
+        for storage in ("f32", "f16"):
+            assert abs(fast[storage][q] - f64) <= 0.05, (q, storage, fast[storage][q], f64)
+            assert abs(fast[storage][q] - want) < 0.25, (q, storage, fast[storage][q], want)
