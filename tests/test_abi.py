@@ -54,3 +54,13 @@ def test_resident_reference_argument_validation_without_device():
     assert L.ssim2_server_add_reference(None, ctypes.byref(img), ctypes.byref(rid)) == 8
     assert L.ssim2_server_remove_reference(None, 1) == 8
     assert L.ssim2_server_submit_ref(None, 1, ctypes.byref(img), 0) == 8
+
+
+def test_bf16_storage_refused_without_device():
+    import ssimulacra2_hip as s2
+    L = s2.lib
+    o = s2._BatchOptions(ctypes.sizeof(s2._BatchOptions), 2, 4, 0, 100_000, 0)
+    out = ctypes.c_void_p()
+    assert L.ssim2_batch_create(ctypes.byref(o), ctypes.byref(out)) == 8
+    assert b"bfloat16" in L.ssim2_last_error()
+    assert not out.value

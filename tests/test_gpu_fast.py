@@ -174,24 +174,31 @@ def test_identical_pairs_score_100(s2, synth):
         assert b.compute([(a, a)])[0] == 100.0
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
-def test_reduced_storage(s2, oracle, mix, storage):
+def test_f16_storage(s2, oracle, mix):
     host, dev = mix
-    _say(f"[gpu] fast batch {storage}")
-    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000, storage=storage) as b:
+    _say("[gpu] fast batch f16")
+    with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000, storage="f16") as b:
         scores = b.compute(dev)
     worst = 0.0
     for (r, d), sc, case in zip(host, scores, MIX):
         f64 = oracle.score_srgb(r, d, F64)
         worst = max(worst, abs(sc - f64))
-        if storage == "f16":
-            same = oracle.score_srgb(r, d, FAST_SHIFT_F16)
-            assert abs(sc - same) <= 5e-3, (case, sc, same)
-        # the published bar is for FP16 storage at the quantiser steps of the
-        # workloads (>= 4); bfloat16 keeps 8 mantissa bits and is reported only
-        if storage == "f16":
-            assert abs(sc - f64) <= BAR_F64, (case, sc, f64)
-    _say(f"[gpu] {storage}: worst |d| vs f64 {worst:.4f}")
+        same = oracle.score_srgb(r, d, FAST_SHIFT_F16)
+        assert abs(sc - same) <= 5e-3, (case, sc, same)
+        assert abs(sc - f64) <= BAR_F64, (case, sc, f64)
+    _say(f"[gpu] f16: worst |d| vs f64 {worst:.4f}")
+
+
+def test_bf16_storage_is_refused(s2):
+    """bfloat16 planes were measured 0.30-0.35 from FP64 on 4K frames: the
+    batch engine, the server and the video driver refuse them."""
+    with pytest.raises(s2.Ssim2Error) as ei:
+        s2.Ssim2Batch(max_pairs=4, max_pixels=100_000, storage="bf16")
+    assert ei.value.status == "SSIM2_INVALID_ARGUMENT" and "bfloat16" in str(ei.value)
+    with pytest.raises(s2.Ssim2Error) as ei:
+        s2.Ssim2Server(storage="bf16", slots=1, max_pairs_per_batch=4, max_pixels_per_batch=100_000,
+                       hbm_budget=1 << 30)
+    assert ei.value.status == "SSIM2_INVALID_ARGUMENT"
 
 
 def test_autotune_keeps_scores(s2, mix):

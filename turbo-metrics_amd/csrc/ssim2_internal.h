@@ -29,7 +29,6 @@ constexpr int kThreads = 256;
 enum Storage : int {
   kStorageF32 = 0,
   kStorageF16 = 1,
-  kStorageBF16 = 2,
 };
 
 // Output tile shapes of the fast form (k_fast_tile instantiations).
