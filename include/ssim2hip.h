@@ -424,6 +424,13 @@ ssim2_status ssim2_time_kernels(ssim2_ctx* ctx, const ssim2_image* ref_srgb,
                                 const ssim2_image* dis_srgb, ssim2_stream stream, int reps,
                                 float* ms);
 
+/* Diagnostic, no counterpart in the reference: evaluates the kernels' cube
+ * root (glibc's cbrtf method with the double division done by reciprocal and
+ * Newton steps) and the same root with the IEEE double division, on the
+ * device, for every float whose bit pattern lies in [lo_bits, hi_bits], and
+ * returns how many results differ. Synchronous. */
+ssim2_status ssim2_check_cube_root(uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches);
+
 /* ---- utilities ------------------------------------------------------------ */
 
 /* Raw device memory (cudarse-driver async alloc, mem.rs:27-35) and a plain

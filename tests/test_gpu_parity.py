@@ -244,3 +244,16 @@ def test_jpeg_corpus_through_gpu_paths(s2, oracle):
         for storage in ("f32", "f16"):
             assert abs(fast[storage][q] - f64) <= 0.05, (q, storage, fast[storage][q], f64)
             assert abs(fast[storage][q] - want) < 0.25, (q, storage, fast[storage][q], want)
+
+
+def test_cube_root_reciprocal_division_on_every_float(s2):
+    """The kernels' cube root divides by reciprocal and Newton steps; on the
+    device it equals the same root with the IEEE double division for every
+    float in [0, 2] (the host build, with the IEEE division, equals the
+    platform cbrtf on the same range: tests/test_oracle.py)."""
+    import ctypes
+    lo, hi = 0, int(np.array(2.0, np.float32).view(np.uint32))
+    n = ctypes.c_uint64()
+    _say("[gpu] cube root: reciprocal vs IEEE division on every float in [0, 2]")
+    s2._check(s2.lib.ssim2_check_cube_root(lo, hi, ctypes.byref(n)))
+    assert n.value == 0, n.value

@@ -11,6 +11,7 @@ typedef __attribute__((address_space(1))) const unsigned short gin_u16;
 typedef __attribute__((address_space(1))) const float gin_f32;
 
 typedef __attribute__((address_space(3))) const float lds_cfloat;
+typedef __attribute__((address_space(3))) const double lds_cdouble;
 
 // Linear RGB of pixel (gx, gy). `lut` is the sRGB u8 table (constant memory or
 // an LDS copy). Chroma of 4:2:0 inputs is taken from the sample of the pixel's

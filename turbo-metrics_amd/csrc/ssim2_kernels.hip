@@ -31,6 +31,7 @@ namespace ssim2 {
 
 __constant__ float c_lut[256] = SSIM2_SRGB8_LUT_INIT;
 __constant__ double c_weight[108] = SSIM2_WEIGHTS_INIT;
+__constant__ double c_cbrt_factor[5] = SSIM2_CBRT_FACTORS_INIT;
 
 namespace {
 
@@ -124,12 +125,15 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 
   // the sRGB table in LDS: the lookups are divergent, so constant memory would
   // serve them as one global load each
+  // and the cube root's factor table (a divergent index too)
   __shared__ float lut_s[256];
-  if (in.kind == kInputSrgbU8) {
+  __shared__ double cbrt_factor_s[5];
+  if (in.kind == kInputSrgbU8)
     for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut[i];
-    lds_barrier();
-  }
+  if (tid < 5) cbrt_factor_s[tid] = c_cbrt_factor[tid];
+  lds_barrier();
   lds_cfloat* lut = (lds_cfloat*)lut_s;
+  const lds_cdouble* cbrt_factor = (const lds_cdouble*)cbrt_factor_s;
 
   // Scales 0 and 1 without a scale-0 tile: each thread takes the 2x2 scale-0
   // block behind its scale-1 pixel (clamped at the image edge, as the
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
         }
       }
       float X, Y, B;
-      linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
+      linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_factor);
       const long long off = (long long)gy * P.sc[0].pitch + gx;
       gbl(xyb_plane(P, 0, img, 0, SIDES))[off] = X;
       gbl(xyb_plane(P, 0, img, 1, SIDES))[off] = Y;
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
       }
       if (gx < P.sc[1].w && gy < P.sc[1].h) {
         float X, Y, B;
-        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
+        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_factor);
         const long long off = (long long)gy * P.sc[1].pitch + gx;
         gbl(xyb_plane(P, 1, img, 0, SIDES))[off] = X;
         gbl(xyb_plane(P, 1, img, 1, SIDES))[off] = Y;
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
             bufA[im * 3 + c][ly][lx] = rgb[c];
         }
         float X, Y, B;
-        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B);
+        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_factor);
         const long long off = (long long)gy * P.sc[s].pitch + gx;
         gbl(xyb_plane(P, s, img, 0, SIDES))[off] = X;
         gbl(xyb_plane(P, s, img, 1, SIDES))[off] = Y;
@@ -817,6 +821,30 @@ hipError_t launch_cols(const PairDesc* pairs, const WorkItem* items, int nitems,
 hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream) {
   if (npairs <= 0) return hipSuccess;
   return launch_finalize_pairs(pairs, nullptr, npairs, npairs * kMaxScales * 3, npairs, stream);
+}
+
+// Every float whose bits lie in [lo, hi]: the cube root with the reciprocal
+// division (the kernels' form) against the same root with the IEEE division;
+// differing results are counted in *mismatches.
+__global__ __launch_bounds__(kThreads) void k_check_cube_root(unsigned lo, unsigned hi,
+                                                              unsigned long long* __restrict__ mismatches) {
+  __shared__ double factor_s[5];
+  if (threadIdx.x < 5) factor_s[threadIdx.x] = c_cbrt_factor[threadIdx.x];
+  lds_barrier();
+  const lds_cdouble* factor = (const lds_cdouble*)factor_s;
+  unsigned long long bad = 0;
+  const unsigned long long n = (unsigned long long)hi - lo + 1;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * kThreads) {
+    const float x = __uint_as_float(lo + (unsigned)i);
+    bad += __float_as_uint(cbrtf_ref<false>(x, factor)) != __float_as_uint(cbrtf_ref<true>(x, factor));
+  }
+  if (bad) atomicAdd(mismatches, bad);
+}
+
+hipError_t launch_check_cube_root(unsigned lo, unsigned hi, unsigned long long* mismatches, hipStream_t stream) {
+  hipLaunchKernelGGL(k_check_cube_root, dim3(8192), dim3(kThreads), 0, stream, lo, hi, mismatches);
+  return hipGetLastError();
 }
 
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream) {

@@ -43,40 +43,73 @@ constexpr float kC2 = 0.0009f;
 // The reference evaluates f32::cbrt, i.e. the platform libm cbrtf (glibc on
 // Linux). Its result differs from the correctly rounded cube root in about
 // one input in ten, and the score is sensitive to that last bit through the
-// marginally stable IIR. This is the published glibc float method (Alboth and
-// Drepper, 1997): split x = m * 2^e with m in [0.5, 1), a quadratic first
-// guess in double, one Halley step in mixed float/double, then scale by
-// 2^(e/3) and a tabulated 2^((e mod 3)/3). Verified equal to the host cbrtf
-// for every float in [0, 2] (the XYB argument range is [0.0038, 1.0038]).
-SSIM2_HD float cbrtf_ref(float x) {
-  const double kCbrt2 = 1.2599210498948731648;     // 2^(1/3)
-  const double kSqrCbrt2 = 1.5874010519681994748;  // 2^(2/3)
+// marginally stable IIR. This restates glibc's float cube root
+// (sysdeps/ieee754/flt-32/s_cbrtf.c, Ulrich Drepper 1997, part of the GNU C
+// Library, LGPL-2.1-or-later; its polynomial and factor constants are used
+// here): split x = m * 2^e with m in [0.5, 1), a quadratic first guess u in
+// double rounded to float, one Halley step u (u^3 + 2m) / (2u^3 + m) in
+// double, times 2^(r/3) for r = e % 3 (C remainder), rounded to float and
+// scaled by 2^(e/3). Equal to the host cbrtf for every float in [0, 2] (the
+// XYB argument range is [0.0038, 1.0038]).
+
+// glibc's factor table 2^(r/3), r = -2 .. 2, at index r + 2
+#define SSIM2_CBRT_FACTORS_INIT \
+  { 1.0 / 1.5874010519681994748, 1.0 / 1.2599210498948731648, 1.0, 1.2599210498948731648, 1.5874010519681994748 }
+
+// a / b rounded to nearest, for positive normal a and b whose quotient is
+// normal. On the device this is the IEEE double division sequence of the
+// compiler without its range scaling and special-case fix-up (identities for
+// such operands): hardware reciprocal, two Newton steps, the quotient and one
+// remainder correction. kIeeeDiv selects the plain division (the form the
+// reciprocal one is checked against on the GPU).
+template <bool kIeeeDiv = false>
+SSIM2_HD double div_rn(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (!kIeeeDiv) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+  }
+#endif
+  return a / b;
+}
+
+// `factor` indexes like SSIM2_CBRT_FACTORS_INIT (an array in any address
+// space: the kernels keep it in LDS).
+template <bool kIeeeDiv = false, typename Factors>
+SSIM2_HD float cbrtf_ref(float x, const Factors& factor) {
   int e;
   const float m = frexpf(fabsf(x), &e);
-  const float u = (float)(0.492659620528969547 +
-                          (0.697570460207922770 - 0.191502161678719066 * (double)m) * (double)m);
+  const double md = m;
+  const float u = (float)(0.492659620528969547 + (0.697570460207922770 - 0.191502161678719066 * md) * md);
   const float t2 = u * u * u;
-  const int r = e % 3;  // in [-2, 2]
-  // the method's five factors 2^(r/3), as doubles; flat selects, not branches
-  double f = 1.0;
-  f = r == 1 ? kCbrt2 : f;
-  f = r == 2 ? kSqrCbrt2 : f;
-  f = r == -1 ? 1.0 / kCbrt2 : f;
-  f = r == -2 ? 1.0 / kSqrCbrt2 : f;
-  const float y = (float)((double)u * ((double)t2 + 2.0 * (double)m) / (2.0 * (double)t2 + (double)m) * f);
+  const double ud = u, t2d = t2;
+  // (t2 + 2m) and (2 t2 + m): the doublings are exact, so one fma rounds
+  // each sum exactly as the sum of the doubled term does
+  const double q = div_rn<kIeeeDiv>(ud * fma(2.0, md, t2d), fma(2.0, t2d, md));
+  const float y = (float)(q * factor[e % 3 + 2]);
   const float res = ldexpf(x > 0.0f ? y : -y, e / 3);
   // zero, infinity and NaN: x + x (frexpf leaves them with exponent 0)
   return (x == 0.0f || !(x == x) || isinf(x)) ? x + x : res;
 }
 
-// cpu.rs:458-496 px_linear_rgb_to_xyb followed by the positive shift.
-SSIM2_HD void linear_to_xyb(float r, float g, float b, float& X, float& Y, float& B) {
+SSIM2_HD float cbrtf_ref(float x) {
+  const double factor[5] = SSIM2_CBRT_FACTORS_INIT;
+  return cbrtf_ref(x, factor);
+}
+
+// cpu.rs:458-496 px_linear_rgb_to_xyb followed by the positive shift;
+// `factor` is the cube root's factor table (SSIM2_CBRT_FACTORS_INIT).
+template <typename Factors>
+SSIM2_HD void linear_to_xyb(float r, float g, float b, float& X, float& Y, float& B, const Factors& factor) {
   float rg = fmaf(kM00, r, fmaf(kM01, g, fmaf(kM02, b, kB0)));
   float gr = fmaf(kM10, r, fmaf(kM11, g, fmaf(kM12, b, kB0)));
   float bb = fmaf(kM20, r, fmaf(kM21, g, fmaf(kM22, b, kB0)));
-  rg = cbrtf_ref(fmaxf(rg, 0.0f)) - kB0Root;
-  gr = cbrtf_ref(fmaxf(gr, 0.0f)) - kB0Root;
-  bb = cbrtf_ref(fmaxf(bb, 0.0f)) - kB0Root;
+  rg = cbrtf_ref(fmaxf(rg, 0.0f), factor) - kB0Root;
+  gr = cbrtf_ref(fmaxf(gr, 0.0f), factor) - kB0Root;
+  bb = cbrtf_ref(fmaxf(bb, 0.0f), factor) - kB0Root;
   float x = 0.5f * (rg - gr);
   float y = 0.5f * (rg + gr);
   X = fmaf(x, 14.0f, 0.42f);
