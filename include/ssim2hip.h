@@ -424,12 +424,16 @@ ssim2_status ssim2_time_kernels(ssim2_ctx* ctx, const ssim2_image* ref_srgb,
                                 const ssim2_image* dis_srgb, ssim2_stream stream, int reps,
                                 float* ms);
 
-/* Diagnostic, no counterpart in the reference: evaluates the kernels' cube
- * root (glibc's cbrtf method with the double division done by reciprocal and
- * Newton steps) and the same root with the IEEE double division, on the
- * device, for every float whose bit pattern lies in [lo_bits, hi_bits], and
- * returns how many results differ. Synchronous. */
-ssim2_status ssim2_check_cube_root(uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches);
+/* Diagnostic, no counterpart in the reference. The kernels replace three IEEE
+ * operations by shorter sequences that must give the same results: the cube
+ * root's double division (glibc's cbrtf method), the SSIM map's f32 quotient
+ * and the edge map's f64 quotient (reciprocal, Newton steps, remainder
+ * corrections). This evaluates both forms on the device for every float x
+ * whose bit pattern lies in [lo_bits, hi_bits] and returns how many results
+ * differ. which 0: cbrt(x). which 1: 1 - a / x in the SSIM map's arithmetic,
+ * for 64 numerators a. which 2: the edge ratio with x as one absolute
+ * difference and 64 values as the other. Synchronous. */
+ssim2_status ssim2_check_arithmetic(uint32_t which, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches);
 
 /* ---- utilities ------------------------------------------------------------ */
 

@@ -246,14 +246,22 @@ def test_jpeg_corpus_through_gpu_paths(s2, oracle):
             assert abs(fast[storage][q] - want) < 0.25, (q, storage, fast[storage][q], want)
 
 
-def test_cube_root_reciprocal_division_on_every_float(s2):
-    """The kernels' cube root divides by reciprocal and Newton steps; on the
-    device it equals the same root with the IEEE double division for every
-    float in [0, 2] (the host build, with the IEEE division, equals the
-    platform cbrtf on the same range: tests/test_oracle.py)."""
+def _f32_bits(v):
+    return int(np.array(v, np.float32).view(np.uint32))
+
+
+@pytest.mark.parametrize("which,lo,hi,what", [
+    (0, 0.0, 2.0, "cube root, every float in [0, 2]"),
+    (1, 2.0 ** -11, 2.0 ** 30, "SSIM quotient, every denominator in [2^-11, 2^30] x 64 numerators"),
+    (2, 0.0, 1024.0, "edge ratio, every |difference| in [0, 1024] x 64 partners"),
+])
+def test_reduced_arithmetic_equals_ieee_on_the_device(s2, which, lo, hi, what):
+    """The kernels' cube root divides by reciprocal and Newton steps, and the
+    maps' f32 and f64 quotients do the same; on the device each equals its
+    IEEE form on the whole swept range (the host build of the cube root, with
+    the IEEE division, equals the platform cbrtf on [0, 2]: tests/test_oracle.py)."""
     import ctypes
-    lo, hi = 0, int(np.array(2.0, np.float32).view(np.uint32))
     n = ctypes.c_uint64()
-    _say("[gpu] cube root: reciprocal vs IEEE division on every float in [0, 2]")
-    s2._check(s2.lib.ssim2_check_cube_root(lo, hi, ctypes.byref(n)))
+    _say(f"[gpu] reduced arithmetic vs IEEE: {what}")
+    s2._check(s2.lib.ssim2_check_arithmetic(which, _f32_bits(lo), _f32_bits(hi), ctypes.byref(n)))
     assert n.value == 0, n.value

@@ -548,16 +548,16 @@ ssim2_status ssim2_device_copy(void* dst, const void* src, size_t bytes, ssim2_s
   return SSIM2_OK;
 }
 
-ssim2_status ssim2_check_cube_root(uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches) {
-  if (!mismatches || lo_bits > hi_bits) return fail(SSIM2_INVALID_ARGUMENT, "bad check_cube_root arguments");
+ssim2_status ssim2_check_arithmetic(uint32_t which, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches) {
+  if (!mismatches || lo_bits > hi_bits || which > 2) return fail(SSIM2_INVALID_ARGUMENT, "bad check_arithmetic arguments");
   unsigned long long* d = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof *d));
   unsigned long long h = 0;
   hipError_t e = hipMemset(d, 0, sizeof *d);
-  if (e == hipSuccess) e = launch_check_cube_root(lo_bits, hi_bits, d, nullptr);
+  if (e == hipSuccess) e = launch_check_arithmetic((int)which, lo_bits, hi_bits, d, nullptr);
   if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
   (void)hipFree(d);
-  if (e != hipSuccess) return hip_fail(e, "check_cube_root");
+  if (e != hipSuccess) return hip_fail(e, "check_arithmetic");
   *mismatches = h;
   return SSIM2_OK;
 }

@@ -648,20 +648,28 @@ __global__ __launch_bounds__(ColWaves<SIDES>::threads) void k_cols(
   // chunk ahead: the maps would otherwise wait out an HBM latency per chunk.
   constexpr int NM = (RY * CB + NT - 1) / NT;  // map pixels per thread per chunk
   constexpr int NV = SIDES == kSidesDis ? 4 : (SIDES == kSidesBoth ? 2 : 0);
+  static_assert(NT % CB == 0, "a map thread keeps its column in every chunk");
   float mv[NM][NV > 0 ? NV : 1], mv_next[NM][NV > 0 ? NV : 1];
+  // buffer loads: the planes as resources, one 32-bit offset per pixel
+  // (row clamped to the image, column clamped once per thread)
+  const long long plane_bytes = S.plane * 4;
+  const __amdgpu_buffer_rsrc_t r_src = buffer_rsrc((const float*)srcp, plane_bytes);
+  const __amdgpu_buffer_rsrc_t r_dis = buffer_rsrc((const float*)disp, plane_bytes);
+  const __amdgpu_buffer_rsrc_t r_s11 = buffer_rsrc((const float*)cache_s11, plane_bytes);
+  const __amdgpu_buffer_rsrc_t r_mu1 = buffer_rsrc((const float*)cache_mu1, plane_bytes);
+  const int mrow = mt / CB, mcol4 = min(xb + mt % CB, w - 1) * 4, pitch4 = pitch * 4;
   auto load_maps = [&](int y0c, float (*dst)[NV > 0 ? NV : 1]) {
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
-      const int px = mt + NT * m;
-      const int n = min(max(y0c + px / CB, 0), h - 1), col = min(xb + px % CB, w - 1);
-      const long long off = (long long)n * pitch + col;
+      const int n = min(max(y0c + mrow + (NT / CB) * m, 0), h - 1);
+      const unsigned off = (unsigned)(n * pitch4 + mcol4);
       if (NV >= 2) {
-        dst[m][0] = srcp[off];
-        dst[m][1] = disp[off];
+        dst[m][0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_src, off, 0, 0));
+        dst[m][1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_dis, off, 0, 0));
       }
       if (NV == 4) {
-        dst[m][2] = cache_s11[off];
-        dst[m][3] = cache_mu1[off];
+        dst[m][2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_s11, off, 0, 0));
+        dst[m][3] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_mu1, off, 0, 0));
       }
     }
   };
@@ -701,18 +709,21 @@ __global__ __launch_bounds__(ColWaves<SIDES>::threads) void k_cols(
                 s12 = bq[1];
                 mu2 = bq[2];
               }
+              // per-pixel terms as cpu.rs forms them; the fourth powers
+              // are accumulated with one rounding (the sums' order differs
+              // from cpu.rs's anyway)
               const double d = ssim_error(mu1, mu2, s11, s22, s12);
               const double d2 = d * d;
               acc[0] += d;
-              acc[1] += d2 * d2;
+              acc[1] = fma(d2, d2, acc[1]);
               const double d1 = edge_ratio(v[0], mu1, v[1], mu2);
               const double art = d1 > 0.0 ? d1 : 0.0;
               const double det = -d1 > 0.0 ? -d1 : 0.0;
               const double a2 = art * art, e2 = det * det;
               acc[2] += art;
-              acc[3] += a2 * a2;
+              acc[3] = fma(a2, a2, acc[3]);
               acc[4] += det;
-              acc[5] += e2 * e2;
+              acc[5] = fma(e2, e2, acc[5]);
             }
           }
         }
@@ -823,11 +834,26 @@ hipError_t launch_finalize(const PairDesc* pairs, int npairs, hipStream_t stream
   return launch_finalize_pairs(pairs, nullptr, npairs, npairs * kMaxScales * 3, npairs, stream);
 }
 
-// Every float whose bits lie in [lo, hi]: the cube root with the reciprocal
-// division (the kernels' form) against the same root with the IEEE division;
-// differing results are counted in *mismatches.
-__global__ __launch_bounds__(kThreads) void k_check_cube_root(unsigned lo, unsigned hi,
-                                                              unsigned long long* __restrict__ mismatches) {
+// Partners of the swept operand in the division checks (both signs, zero,
+// subnormal, tiny, the workloads' magnitudes and large values).
+__constant__ float c_check_partners[64] = {
+    0.0f,      0x1p-149f,  0x1p-126f,   0x1p-110f,   0x1p-100f,   0x1p-90f,    0x1p-60f,   1e-20f,
+    1e-10f,    1e-7f,      1e-5f,       1e-4f,       0.0009f,     0.001f,      0.0123f,    0.03125f,
+    0.1f,      0.1234567f, 0.25f,       0.3333333f,  0.5f,        0.70710677f, 0.9f,       0.99999994f,
+    1.0f,      1.0000001f, 1.2599211f,  1.5f,        1.9999999f,  2.0f,        3.0f,       3.1415927f,
+    4.0f,      7.389056f,  10.0f,       100.0f,      1e4f,        1e8f,        1e20f,      0x1p100f,
+    -0x1p-149f, -1e-20f,   -1e-7f,      -0.0009f,    -0.001f,     -0.0123f,    -0.1f,      -0.25f,
+    -0.5f,     -0.9f,      -1.0f,       -1.5f,       -2.0f,       -3.0f,       -10.0f,     -100.0f,
+    0.0008999f, 0.00090001f, 0.6180340f, 0.8414710f, 1.7320508f,  2.7182817f,  5.0f,       0.4999999f};
+
+// Diagnostics of the reduced arithmetic against the IEEE forms, on the
+// device, for every float x whose bits lie in [lo, hi]; differing results are
+// counted in *mismatches. which 0: the cube root of x. which 1: the SSIM
+// map's 1 - q for q = a / x in f32, a over the partners (x is the
+// denominator, >= C2 minus rounding in the maps). which 2: the edge ratio with
+// |dis - mu2| = x and |src - mu1| over the partners' magnitudes.
+__global__ __launch_bounds__(kThreads) void k_check_arithmetic(int which, unsigned lo, unsigned hi,
+                                                               unsigned long long* __restrict__ mismatches) {
   __shared__ double factor_s[5];
   if (threadIdx.x < 5) factor_s[threadIdx.x] = c_cbrt_factor[threadIdx.x];
   lds_barrier();
@@ -837,13 +863,28 @@ __global__ __launch_bounds__(kThreads) void k_check_cube_root(unsigned lo, unsig
   for (unsigned long long i = (unsigned long long)blockIdx.x * kThreads + threadIdx.x; i < n;
        i += (unsigned long long)gridDim.x * kThreads) {
     const float x = __uint_as_float(lo + (unsigned)i);
-    bad += __float_as_uint(cbrtf_ref<false>(x, factor)) != __float_as_uint(cbrtf_ref<true>(x, factor));
+    if (which == 0) {
+      bad += __float_as_uint(cbrtf_ref<false>(x, factor)) != __float_as_uint(cbrtf_ref<true>(x, factor));
+    } else if (which == 1) {
+      for (int k = 0; k < 64; ++k) {
+        const float a = c_check_partners[k];
+        const double f = 1.0 - (double)div_rn_f32<false>(a, x), g = 1.0 - (double)div_rn_f32<true>(a, x);
+        bad += __double_as_longlong(f) != __double_as_longlong(g);
+      }
+    } else {
+      for (int k = 0; k < 64; ++k) {
+        const float b = fabsf(c_check_partners[k]);
+        const double f = edge_ratio<false>(b, 0.0f, x, 0.0f), g = edge_ratio<true>(b, 0.0f, x, 0.0f);
+        bad += __double_as_longlong(f) != __double_as_longlong(g);
+      }
+    }
   }
   if (bad) atomicAdd(mismatches, bad);
 }
 
-hipError_t launch_check_cube_root(unsigned lo, unsigned hi, unsigned long long* mismatches, hipStream_t stream) {
-  hipLaunchKernelGGL(k_check_cube_root, dim3(8192), dim3(kThreads), 0, stream, lo, hi, mismatches);
+hipError_t launch_check_arithmetic(int which, unsigned lo, unsigned hi, unsigned long long* mismatches,
+                                   hipStream_t stream) {
+  hipLaunchKernelGGL(k_check_arithmetic, dim3(8192), dim3(kThreads), 0, stream, which, lo, hi, mismatches);
   return hipGetLastError();
 }
 

@@ -117,20 +117,44 @@ SSIM2_HD void linear_to_xyb(float r, float g, float b, float& X, float& Y, float
   B = (bb - y) + 0.55f;
 }
 
+// a / b rounded to nearest in f32 for finite b with |b| in [2^-20, 2^100]
+// and |a / b| >= 2^-100 (smaller quotients come out within 2^-90 of
+// zero, which is all ssim_error needs of them): on the device the compiler's
+// IEEE f32 division sequence without its range scaling and fix-up (hardware
+// reciprocal, one Newton step, the quotient and two remainder corrections).
+template <bool kIeeeDiv = false>
+SSIM2_HD float div_rn_f32(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (!kIeeeDiv) {
+    float r = __builtin_amdgcn_rcpf(b);
+    r = fmaf(fmaf(-b, r, 1.0f), r, r);
+    float q = a * r;
+    q = fmaf(fmaf(-b, q, a), r, q);
+    return fmaf(fmaf(-b, q, a), r, q);
+  }
+#endif
+  return a / b;
+}
+
 // cpu.rs:605-630: 1 - SSIM' for one pixel, f32 ratio, f64 "1 -", clamped at 0.
+// den >= C2 minus the rounding of the variances; a quotient below 2^-90 in
+// magnitude leaves 1 - q equal to 1.0 in double either way.
+template <bool kIeeeDiv = false>
 SSIM2_HD double ssim_error(float mu1, float mu2, float s11, float s22, float s12) {
   float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
   float md = mu1 - mu2;
   float num_m = fmaf(md, -md, 1.0f);
   float num_s = fmaf(2.0f, s12 - mu12, kC2);
   float den = ((s11 - mu11) + (s22 - mu22)) + kC2;
-  double d = 1.0 - (double)((num_m * num_s) / den);
+  double d = 1.0 - (double)div_rn_f32<kIeeeDiv>(num_m * num_s, den);
   return d > 0.0 ? d : 0.0;
 }
 
-// cpu.rs:658-674: edge-difference ratio in f64 (f32 differences).
+// cpu.rs:658-674: edge-difference ratio in f64 (f32 differences). Both
+// operands lie in [1, 2^128) for finite images.
+template <bool kIeeeDiv = false>
 SSIM2_HD double edge_ratio(float src, float mu1, float dis, float mu2) {
-  return (1.0 + (double)fabsf(dis - mu2)) / (1.0 + (double)fabsf(src - mu1)) - 1.0;
+  return div_rn<kIeeeDiv>(1.0 + (double)fabsf(dis - mu2), 1.0 + (double)fabsf(src - mu1)) - 1.0;
 }
 
 // ---- YUV 4:2:0 BT.709 limited range -> linear RGB ------------------------

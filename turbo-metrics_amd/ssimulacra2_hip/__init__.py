@@ -229,7 +229,7 @@ def _load():
         "ssim2_device_free": (vp,),
         "ssim2_device_copy": (vp, vp, sz, vp),
         "ssim2_device_synchronize": (),
-        "ssim2_check_cube_root": (u32, u32, P(u64)),
+        "ssim2_check_arithmetic": (u32, u32, u32, P(u64)),
         "ssim2_batch_create": (P(_BatchOptions), P(vp)),
         "ssim2_batch_destroy": (vp,),
         "ssim2_batch_mem_usage": (vp, P(sz)),
