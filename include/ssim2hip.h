@@ -450,6 +450,17 @@ ssim2_status ssim2_device_synchronize(void);
 ssim2_status ssim2_srgb_to_linear(const ssim2_image* src_u8, ssim2_image* dst_f32,
                                   ssim2_stream stream);
 
+/* Kernel::downscale_by_2 (crates/ssimulacra2-cuda/src/kernel.rs:59-83; kernel
+ * crates/ssimulacra2-cuda-kernel/src/downscale.rs:4-35): packed f32 RGB, 2x2
+ * mean with the source clamped at its last row and column. dst must be
+ * ceil(width/2) x ceil(height/2) of src. */
+ssim2_status ssim2_downscale_by_2(const ssim2_image* src_f32, ssim2_image* dst_f32, ssim2_stream stream);
+
+/* Kernel::linear_to_xyb (crates/ssimulacra2-cuda/src/kernel.rs:161-184; kernel
+ * crates/ssimulacra2-cuda-kernel/src/xyb.rs:81-102): packed f32 linear RGB to
+ * packed positive XYB (X, Y, B per pixel), the scoring path's arithmetic. */
+ssim2_status ssim2_linear_to_xyb(const ssim2_image* src_f32, ssim2_image* dst_f32, ssim2_stream stream);
+
 /* CudaColorspace::biplanaryuv420_to_linearrgb_16 (crates/cuda-colorspace/src/
  * lib.rs:79-123; kernel crates/cuda-colorspace-kernel/src/biplanar.rs:8-70):
  * YUV 4:2:0 BT.709 limited -> packed f32 linear RGB (the power function is

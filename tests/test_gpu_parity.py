@@ -265,3 +265,28 @@ def test_reduced_arithmetic_equals_ieee_on_the_device(s2, which, lo, hi, what):
     _say(f"[gpu] reduced arithmetic vs IEEE: {what}")
     s2._check(s2.lib.ssim2_check_arithmetic(which, _f32_bits(lo), _f32_bits(hi), ctypes.byref(n)))
     assert n.value == 0, n.value
+
+
+def test_stage_kernels_equal_the_oracle(s2, oracle, synth):
+    """Kernel::srgb_to_linear / downscale_by_2 / linear_to_xyb
+    (crates/ssimulacra2-cuda/src/kernel.rs:36-184) through the C ABI, bit-equal
+    to the oracle's cpu.rs stages, on odd sizes."""
+    for w, h in [(37, 23), (64, 64), (1001, 3)]:
+        ref, _ = synth.pair(1, 90 + w, w, h)
+        _say(f"[gpu] stage kernels {w}x{h}")
+        src = s2.DeviceImage.from_host(ref)
+        lin = s2.DeviceImage(w, h, np.float32)
+        k = s2.Kernel()
+        k.srgb_to_linear(src, lin)
+        want_lin = oracle.lut()[ref]
+        np.testing.assert_array_equal(lin.download(), want_lin)
+        half = s2.DeviceImage((w + 1) // 2, (h + 1) // 2, np.float32)
+        k.downscale_by_2(lin, half)
+        np.testing.assert_array_equal(half.download(), oracle.downscale(want_lin))
+        xyb = s2.DeviceImage(w, h, np.float32)
+        k.linear_to_xyb(lin, xyb)
+        np.testing.assert_array_equal(xyb.download(), oracle.xyb_f32(want_lin))
+    bad = s2.DeviceImage(10, 10, np.float32)
+    with pytest.raises(s2.Ssim2Error) as ei:
+        s2.downscale_by_2(s2.DeviceImage(37, 23, np.float32), bad)
+    assert ei.value.status == "SSIM2_DIM_MISMATCH"

@@ -598,6 +598,36 @@ ssim2_status ssim2_srgb_to_linear(const ssim2_image* src, ssim2_image* dst, ssim
   return SSIM2_OK;
 }
 
+static ssim2_status check_f32_image(const ssim2_image* im, const char* name) {
+  if (!im || !im->ptr) return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + " is null");
+  if (im->width == 0 || im->height == 0) return fail(SSIM2_INVALID_SIZE, std::string(name) + " is empty");
+  if (im->pitch_bytes < size_t(im->width) * 12 || im->pitch_bytes % 4 || (reinterpret_cast<uintptr_t>(im->ptr) & 3))
+    return fail(SSIM2_INVALID_ARGUMENT, std::string(name) + " pitch too small or misaligned");
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_downscale_by_2(const ssim2_image* src, ssim2_image* dst, ssim2_stream stream) {
+  ssim2_status st;
+  if ((st = check_f32_image(src, "src")) != SSIM2_OK || (st = check_f32_image(dst, "dst")) != SSIM2_OK) return st;
+  if (dst->width != (src->width + 1) / 2 || dst->height != (src->height + 1) / 2)
+    return fail(SSIM2_DIM_MISMATCH, "dst must be ceil(width / 2) x ceil(height / 2) of src");
+  HIP_TRY(launch_downscale_by_2(static_cast<const float*>(src->ptr), (long long)src->pitch_bytes, (int)src->width,
+                                (int)src->height, static_cast<float*>(dst->ptr), (long long)dst->pitch_bytes,
+                                (int)dst->width, (int)dst->height, (hipStream_t)stream));
+  return SSIM2_OK;
+}
+
+ssim2_status ssim2_linear_to_xyb(const ssim2_image* src, ssim2_image* dst, ssim2_stream stream) {
+  ssim2_status st;
+  if ((st = check_f32_image(src, "src")) != SSIM2_OK || (st = check_f32_image(dst, "dst")) != SSIM2_OK) return st;
+  if (src->width != dst->width || src->height != dst->height)
+    return fail(SSIM2_DIM_MISMATCH, "source and destination sizes differ");
+  HIP_TRY(launch_linear_to_xyb(static_cast<const float*>(src->ptr), (long long)src->pitch_bytes,
+                               static_cast<float*>(dst->ptr), (long long)dst->pitch_bytes, (int)src->width,
+                               (int)src->height, (hipStream_t)stream));
+  return SSIM2_OK;
+}
+
 ssim2_status ssim2_yuv420_to_linear(const ssim2_yuv_image* src, ssim2_image* dst, ssim2_stream stream) {
   if (!src || !dst || !dst->ptr) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
   ssim2_status st = check_yuv(src, dst->width, dst->height, "src");

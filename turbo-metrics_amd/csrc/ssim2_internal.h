@@ -167,6 +167,10 @@ hipError_t launch_finalize_pairs(const PairDesc* pairs, const int* meta, int npa
 hipError_t launch_yuv_to_linear(const InputView& in, float* out, long long out_pitch, int w, int h,
                                 hipStream_t stream);
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream);
+hipError_t launch_downscale_by_2(const float* src, long long src_pitch, int sw, int sh, float* dst,
+                                 long long dst_pitch, int dw, int dh, hipStream_t stream);
+hipError_t launch_linear_to_xyb(const float* src, long long src_pitch, float* dst, long long dst_pitch, int w, int h,
+                                hipStream_t stream);
 hipError_t launch_check_arithmetic(int which, unsigned lo, unsigned hi, unsigned long long* mismatches,
                                    hipStream_t stream);
 hipError_t launch_srgb_to_linear(const unsigned char* in, long long in_pitch, float* out,

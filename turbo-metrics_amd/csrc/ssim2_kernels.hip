@@ -888,6 +888,66 @@ hipError_t launch_check_arithmetic(int which, unsigned lo, unsigned hi, unsigned
   return hipGetLastError();
 }
 
+// Kernel::downscale_by_2 (crates/ssimulacra2-cuda-kernel/src/downscale.rs:4-35,
+// cpu.rs:545-579): one thread per output pixel, the 2x2 mean of packed f32
+// RGB with the source coordinates clamped to the last row and column, summed
+// in the order (x0,y0), (x1,y0), (x0,y1), (x1,y1).
+__global__ __launch_bounds__(kThreads) void k_downscale_by_2(const float* __restrict__ src, long long src_pitch,
+                                                             int sw, int sh, float* __restrict__ dst,
+                                                             long long dst_pitch, int dw, int dh) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= dw || y >= dh) return;
+  const int x0 = min(2 * x, sw - 1), x1 = min(2 * x + 1, sw - 1);
+  const int y0 = min(2 * y, sh - 1), y1 = min(2 * y + 1, sh - 1);
+  gcfloat* r0 = gbl(const_cast<float*>(reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(src) + y0 * src_pitch)));
+  gcfloat* r1 = gbl(const_cast<float*>(reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(src) + y1 * src_pitch)));
+  gfloat* o = gbl(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(dst) + y * dst_pitch)) + 3 * x;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float sum = 0.0f;
+    sum += r0[3 * x0 + c];
+    sum += r0[3 * x1 + c];
+    sum += r1[3 * x0 + c];
+    sum += r1[3 * x1 + c];
+    o[c] = sum * 0.25f;
+  }
+}
+
+// Kernel::linear_to_xyb (crates/ssimulacra2-cuda-kernel/src/xyb.rs:81-102,
+// cpu.rs:458-496): packed f32 linear RGB to packed positive XYB, one thread
+// per pixel, the pyramid's arithmetic (cube root with its factor table in LDS).
+__global__ __launch_bounds__(kThreads) void k_linear_to_xyb(const float* __restrict__ src, long long src_pitch,
+                                                            float* __restrict__ dst, long long dst_pitch, int w,
+                                                            int h) {
+  __shared__ double factor_s[5];
+  if (threadIdx.x < 5) factor_s[threadIdx.x] = c_cbrt_factor[threadIdx.x];
+  lds_barrier();
+  const lds_cdouble* factor = (const lds_cdouble*)factor_s;
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  gcfloat* in = gbl(const_cast<float*>(reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(src) + y * src_pitch))) + 3 * x;
+  gfloat* o = gbl(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(dst) + y * dst_pitch)) + 3 * x;
+  float X, Y, B;
+  linear_to_xyb(in[0], in[1], in[2], X, Y, B, factor);
+  o[0] = X;
+  o[1] = Y;
+  o[2] = B;
+}
+
+hipError_t launch_downscale_by_2(const float* src, long long src_pitch, int sw, int sh, float* dst,
+                                 long long dst_pitch, int dw, int dh, hipStream_t stream) {
+  hipLaunchKernelGGL(k_downscale_by_2, dim3((dw + 63) / 64, (dh + 3) / 4), dim3(kThreads), 0, stream, src, src_pitch,
+                     sw, sh, dst, dst_pitch, dw, dh);
+  return hipGetLastError();
+}
+
+hipError_t launch_linear_to_xyb(const float* src, long long src_pitch, float* dst, long long dst_pitch, int w, int h,
+                                hipStream_t stream) {
+  hipLaunchKernelGGL(k_linear_to_xyb, dim3((w + 63) / 64, (h + 3) / 4), dim3(kThreads), 0, stream, src, src_pitch,
+                     dst, dst_pitch, w, h);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(const void* src, void* dst, long long bytes, hipStream_t stream) {
   const long long n16 = bytes / 16;
   if (n16 <= 0) return hipSuccess;

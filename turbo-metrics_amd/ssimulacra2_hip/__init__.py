@@ -207,6 +207,8 @@ def _load():
         "ssim2_has_reference": (vp, P(ctypes.c_int)),
         "ssim2_get_sums": (vp, P(dbl), P(ctypes.c_int), P(ctypes.c_int)),
         "ssim2_srgb_to_linear": (img, img, vp),
+        "ssim2_downscale_by_2": (img, img, vp),
+        "ssim2_linear_to_xyb": (img, img, vp),
         "ssim2_yuv420_to_linear": (P(_Yuv), img, vp),
         "ssim2_compute_yuv420_async": (vp, P(_Yuv), P(_Yuv), vp),
         "ssim2_compute_yuv420_sync": (vp, P(_Yuv), P(_Yuv), vp, P(dbl)),
@@ -465,6 +467,25 @@ def srgb_to_linear(src: DeviceImage, dst: DeviceImage, stream: Stream | None = N
     _check(lib.ssim2_srgb_to_linear(src.as_struct(), dst.as_struct(), _stream_ptr(stream)))
 
 
+def downscale_by_2(src: DeviceImage, dst: DeviceImage, stream: Stream | None = None):
+    """Kernel::downscale_by_2 (crates/ssimulacra2-cuda/src/kernel.rs:59-83): f32 images, dst half size (rounded up)."""
+    _check(lib.ssim2_downscale_by_2(src.as_struct(), dst.as_struct(), _stream_ptr(stream)))
+
+
+def linear_to_xyb(src: DeviceImage, dst: DeviceImage, stream: Stream | None = None):
+    """Kernel::linear_to_xyb (crates/ssimulacra2-cuda/src/kernel.rs:161-184): f32 linear RGB to positive XYB."""
+    _check(lib.ssim2_linear_to_xyb(src.as_struct(), dst.as_struct(), _stream_ptr(stream)))
+
+
+class Kernel:
+    """Ssimulacra2::kernel() (crates/ssimulacra2-cuda/src/lib.rs:199-204): the
+    individual stage kernels, as module functions."""
+
+    srgb_to_linear = staticmethod(lambda src, dst, stream=None: srgb_to_linear(src, dst, stream))
+    downscale_by_2 = staticmethod(lambda src, dst, stream=None: downscale_by_2(src, dst, stream))
+    linear_to_xyb = staticmethod(lambda src, dst, stream=None: linear_to_xyb(src, dst, stream))
+
+
 class Ssimulacra2:
     """ssimulacra2_cuda::Ssimulacra2 for MI355X (crates/ssimulacra2-cuda/src/lib.rs:27-602).
 
@@ -607,6 +628,10 @@ class Ssimulacra2:
         v = ctypes.c_int()
         _check(lib.ssim2_has_reference(self._ctx, ctypes.byref(v)))
         return bool(v.value)
+
+    def kernel(self) -> "Kernel":
+        """Ssimulacra2::kernel (lib.rs:199-204)."""
+        return Kernel()
 
     # -- diagnostics
     def time_kernels(self, src_ref: DeviceImage, src_dis: DeviceImage, reps: int = 10, stream=None):
