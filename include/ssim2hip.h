@@ -415,6 +415,15 @@ ssim2_status ssim2_video_compute_all(uint32_t width, uint32_t height, const ssim
  * the last computation. Any pointer may be NULL. */
 ssim2_status ssim2_get_sums(ssim2_ctx* ctx, double* sums108, int* dims12, int* nscales);
 
+/* Copies one intermediate plane of the last computation to the host (waits
+ * for it): kind 0 = XYB plane `index` (0..2 reference X, Y, B; 3..5 the
+ * distorted image's), kind 1 = row-pass plane `index` = channel * 5 +
+ * quantity (reference^2, distorted^2, reference*distorted, reference,
+ * distorted), at scale `scale`; rows of the scale's width, `host_row_floats`
+ * apart. No counterpart in the reference; for stage-by-stage parity checks. */
+ssim2_status ssim2_get_plane(ssim2_ctx* ctx, uint32_t kind, uint32_t scale, uint32_t index, float* host,
+                             size_t host_row_floats);
+
 /* Per-kernel timing of the sRGB u8 pipeline: runs it `reps` times with HIP
  * events around each of its four kernels on `stream` and writes the mean
  * milliseconds per launch to ms[4] = {pyramid, row pass, column pass + maps,

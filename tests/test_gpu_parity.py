@@ -290,3 +290,33 @@ def test_stage_kernels_equal_the_oracle(s2, oracle, synth):
     with pytest.raises(s2.Ssim2Error) as ei:
         s2.downscale_by_2(s2.DeviceImage(37, 23, np.float32), bad)
     assert ei.value.status == "SSIM2_DIM_MISMATCH"
+
+
+@pytest.mark.parametrize("w,h", [(300, 200), (97, 150), (1920, 1080)])
+def test_intermediate_planes_equal_the_oracle(s2, oracle, synth, w, h):
+    """Every XYB plane and every row-pass plane of every scale, bit-equal to
+    the oracle's stages (cpu.rs: downscale of the linear planes, XYB, the
+    row-direction IIR of the five quantities). The row pass stores through
+    16-byte buffer stores whose data registers need the compiler's wait
+    states; this compares those stores' planes directly."""
+    ref, dis = synth.pair(2, 7 + w, w, h, 12)
+    lut = oracle.lut()
+    lin = [lut[ref], lut[dis]]
+    _say(f"[gpu] intermediate planes {w}x{h}")
+    with s2.Ssimulacra2(w, h) as m:
+        m.compute_from_cpu_srgb_sync(ref, dis)
+        dims = oracle.scale_dims(w, h)
+        for s, (sw, sh) in enumerate(dims):
+            if s > 0:
+                lin = [oracle.downscale(x) for x in lin]
+            xyb = [oracle.xyb_f32(x) for x in lin]
+            for img in range(2):
+                for c in range(3):
+                    np.testing.assert_array_equal(m.plane("xyb", s, 3 * img + c), xyb[img][..., c],
+                                                  err_msg=f"xyb scale {s} image {img} channel {c}")
+            for c in range(3):
+                r, d = xyb[0][..., c], xyb[1][..., c]
+                for q, qv in enumerate((r * r, d * d, r * d, r, d)):
+                    want = oracle.plane_op("oracle_blur_rows_f32", qv.astype(np.float32))
+                    np.testing.assert_array_equal(m.plane("rows", s, 5 * c + q), want,
+                                                  err_msg=f"rows scale {s} channel {c} quantity {q}")

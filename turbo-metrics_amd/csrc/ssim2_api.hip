@@ -498,6 +498,21 @@ ssim2_status ssim2_get_sums(ssim2_ctx* c, double* sums, int* dims, int* nscales)
   return SSIM2_OK;
 }
 
+ssim2_status ssim2_get_plane(ssim2_ctx* c, uint32_t kind, uint32_t scale, uint32_t index, float* host,
+                             size_t host_row_floats) {
+  if (!c || !host) return fail(SSIM2_INVALID_ARGUMENT, "null argument");
+  const PairDesc& d = c->plan.desc;
+  if ((int)scale >= d.nscales) return fail(SSIM2_INVALID_ARGUMENT, "scale beyond the pair's scale count");
+  if (kind > 1 || index >= (kind == 0 ? 6u : 3u * kNumQ)) return fail(SSIM2_INVALID_ARGUMENT, "bad plane kind or index");
+  const ScaleDesc& S = d.sc[scale];
+  if (host_row_floats < size_t(S.w)) return fail(SSIM2_INVALID_ARGUMENT, "host rows shorter than the plane's width");
+  if (c->issued) HIP_TRY(hipEventSynchronize(c->done));
+  const float* src = c->arena + (kind == 0 ? S.xyb : S.hrow) + (long long)index * S.plane;
+  HIP_TRY(hipMemcpy2D(host, host_row_floats * sizeof(float), src, size_t(S.pitch) * sizeof(float),
+                      size_t(S.w) * sizeof(float), S.h, hipMemcpyDeviceToHost));
+  return SSIM2_OK;
+}
+
 ssim2_status ssim2_time_kernels(ssim2_ctx* c, const ssim2_image* ref, const ssim2_image* dis,
                                 ssim2_stream stream, int reps, float* ms) {
   if (!c || !ms || reps <= 0) return fail(SSIM2_INVALID_ARGUMENT, "bad time_kernels arguments");

@@ -296,6 +296,8 @@ extern "C" {
         -> ssim2_status;
     pub fn ssim2_time_kernels(ctx: *mut ssim2_ctx, ref_srgb: *const ssim2_image, dis_srgb: *const ssim2_image,
                               stream: ssim2_stream, reps: c_int, ms: *mut f32) -> ssim2_status;
+    pub fn ssim2_get_plane(ctx: *mut ssim2_ctx, kind: u32, scale: u32, index: u32, host: *mut f32,
+                           host_row_floats: usize) -> ssim2_status;
     pub fn ssim2_check_arithmetic(which: u32, lo_bits: u32, hi_bits: u32, mismatches: *mut u64) -> ssim2_status;
 
     // utilities

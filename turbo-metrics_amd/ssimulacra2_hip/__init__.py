@@ -206,6 +206,7 @@ def _load():
         "ssim2_clear_reference": (vp,),
         "ssim2_has_reference": (vp, P(ctypes.c_int)),
         "ssim2_get_sums": (vp, P(dbl), P(ctypes.c_int), P(ctypes.c_int)),
+        "ssim2_get_plane": (vp, u32, u32, u32, vp, sz),
         "ssim2_srgb_to_linear": (img, img, vp),
         "ssim2_downscale_by_2": (img, img, vp),
         "ssim2_linear_to_xyb": (img, img, vp),
@@ -640,6 +641,17 @@ class Ssimulacra2:
         _check(lib.ssim2_time_kernels(self._ctx, src_ref.as_struct(), src_dis.as_struct(),
                                       _stream_ptr(stream), reps, ms))
         return dict(zip(("pyramid", "rows", "cols", "finalize"), [float(v) for v in ms]))
+
+    def plane(self, kind: str, scale: int, index: int) -> np.ndarray:
+        """An intermediate f32 plane of the last computation: kind "xyb"
+        (index 0..5: reference X, Y, B, distorted X, Y, B) or "rows" (index
+        channel * 5 + quantity: ref^2, dis^2, ref*dis, ref, dis)."""
+        w, h = self.width, self.height
+        for _ in range(scale):
+            w, h = (w + 1) // 2, (h + 1) // 2
+        out = np.empty((h, w), np.float32)
+        _check(lib.ssim2_get_plane(self._ctx, {"xyb": 0, "rows": 1}[kind], scale, index, out.ctypes.data, w))
+        return out
 
     def sums(self):
         """(sums[6,3,6], dims[6,2], nscales) behind the last score."""
