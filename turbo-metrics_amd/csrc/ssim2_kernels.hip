@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 // the ring. The buffer range check turns reads of rows past the image into 0
 // and drops stores outside the band's rows.
 constexpr int kChunk = 32;                      // columns per staged chunk (8 groups of 4)
-constexpr int kRowPrefetch = 1;                 // chunks of row-pass input in flight per wave
+constexpr int kRowPrefetch = 2;                 // chunks of row-pass input in flight per wave
 // LDS tiles: rows of 8 float4 (one chunk) with segment g of row r in slot
 // g ^ (r & 7), and the three tiles 3 float4 apart beyond their rows (the
 // first of those is the slot where lanes past a chunk park). LDS serves 16
@@ -339,10 +339,12 @@ __device__ __forceinline__ float4 row_quantity(float4 x, float4 y) {
 //
 // Lanes: each 16-lane row of the wave holds five rows of the band as lane
 // triples (lane 15 idle); lane 3t+k runs cosine term k of its row, so a lane
-// issues one recurrence (4 instructions) per column. Each lane writes its
-// term's outputs to the LDS tile of that term; the coalesced store phase
-// reads the three tiles and forms (o1 + o3) + o5 (cpu.rs's order) once per
-// output, instead of a cross-lane sum per column step.
+// issues one recurrence (4 instructions) per column. The quantity (a square,
+// a product or a plane itself) is formed once per element in the coalesced
+// layout and staged in one tile. Each lane writes its term's outputs to the
+// LDS tile of that term; the coalesced store phase reads the three tiles and
+// forms (o1 + o3) + o5 (cpu.rs's order) once per output, one chunk later, so
+// the LDS round trip of the outputs overlaps the next chunk's scan.
 //
 // The buffer range check applies to the per-lane offset only, so rows past
 // the image carry an out-of-range per-lane offset and the column is the
@@ -353,7 +355,7 @@ __device__ __forceinline__ vf4 to_vf4(float4 v) { return vf4{v.x, v.y, v.z, v.w}
 template <int Q>
 __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buffer_rsrc_t rb,
                                          __amdgpu_buffer_rsrc_t ro, int w, int pitch, int nrows,
-                                         lds_vf4* tile_a, lds_vf4* tile_b, lds_vf4* tile_c) {
+                                         lds_vf4* tile_x, lds_vf4* tile_o) {
   constexpr bool NA = Q != 1 && Q != 4;
   constexpr bool NB = Q == 1 || Q == 2 || Q == 4;
   constexpr int NE = kRowsPerWave * 8;       // float4 of one chunk: rows x 8 segments
@@ -362,7 +364,10 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
   const int dl = lane & 15, tr = min(dl / 3, 4), k = dl - 3 * (dl / 3);
   const int r = 5 * (lane >> 4) + tr;        // this lane's row in the band
   const bool active = dl < 15;               // lane 15 of each 16 duplicates lane 12
-  lds_vf4* tile_k = k == 0 ? tile_a : (k == 1 ? tile_b : tile_c);  // this lane's term outputs
+  lds_vf4* tile_k = tile_o + k * kTileF4;    // this lane's term outputs
+  lds_vf4* tile_o1 = tile_o;
+  lds_vf4* tile_o3 = tile_o + kTileF4;
+  lds_vf4* tile_o5 = tile_o + 2 * kTileF4;
   const float mi = k == 0 ? kMulIn1 : (k == 1 ? kMulIn3 : kMulIn5);
   const float mp = k == 0 ? kMulPrev1 : (k == 1 ? kMulPrev3 : kMulPrev5);
   const u32x4 z = {0u, 0u, 0u, 0u};
@@ -432,58 +437,17 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
   step(0.f, ring[0].y);
   step(0.f, ring[0].z);
   step(0.f, ring[0].w);
-  auto chunk = [&](int kc, u32x4(&da)[NI], u32x4(&db)[NI]) {
-    // This chunk's loads are done once at most the later chunks' loads are
-    // outstanding: loads complete in issue order, but stores (some of them
-    // wholly out of range) may complete before earlier loads, so the wait
-    // must not count on the stores issued in between.
-    __builtin_amdgcn_s_waitcnt(vmcnt_only<(kRowPrefetch - 1) * NI * (int(NA) + int(NB))>());
-    // stage x columns 32k+4 .. 32k+35 of the band's rows into the padded tiles
-#pragma unroll
-    for (int i = 0; i < NI; ++i) {
-      if (NA) tile_a[eidx[i]] = to_vf4(u2f(da[i]));
-      if (NB) tile_b[eidx[i]] = to_vf4(u2f(db[i]));
-    }
-    wave_lds_order();
-    load_chunk(min(kc + kRowPrefetch, nchunks - 1), da, db);
-    float4 xa[8], xb[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      xa[j] = z4;
-      xb[j] = z4;
-      if (NA) xa[j] = to_f4(tile_a[rslot[j]]);
-      if (NB) xb[j] = to_f4(tile_b[rslot[j]]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    float4 out[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      ring[(j + 1) & 3] = row_quantity<Q>(xa[j], xb[j]);  // x-group 8k+j+1
-      const float4 rr = ring[(j + 1) & 3], m2 = ring[(j + 2) & 3], m1 = ring[(j + 3) & 3];
-      out[j].x = step(m2.z, rr.x);
-      out[j].y = step(m2.w, rr.y);
-      out[j].z = step(m1.x, rr.z);
-      out[j].w = step(m1.y, rr.w);
-    }
-    // back through the tiles: every term's rows in, coalesced rows out with
-    // the terms summed; segments past the last output group are dropped with
-    // an out-of-range offset
-    wave_lds_order();
-    if (active) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) tile_k[rslot[j]] = to_vf4(out[j]);
-    }
-    wave_lds_order();
+  // stores of chunk kc from the term tiles (kc < 0: nothing)
+  auto store_chunk = [&](int kc) {
     float4 v[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-      const float4 o1 = to_f4(tile_a[eidx[i]]), o3 = to_f4(tile_b[eidx[i]]), o5 = to_f4(tile_c[eidx[i]]);
+      const float4 o1 = to_f4(tile_o1[eidx[i]]), o3 = to_f4(tile_o3[eidx[i]]), o5 = to_f4(tile_o5[eidx[i]]);
       v[i] = make_float4((o1.x + o3.x) + o5.x, (o1.y + o3.y) + o5.y, (o1.z + o3.z) + o5.z, (o1.w + o3.w) + o5.w);
     }
-    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-      const bool seg_ok = 8 * kc + ((64 * i + lane) & 7) <= glast;
+      const bool seg_ok = kc >= 0 && 8 * kc + ((64 * i + lane) & 7) <= glast;
       const u32x4 u = {__float_as_uint(v[i].x), __float_as_uint(v[i].y), __float_as_uint(v[i].z),
                        __float_as_uint(v[i].w)};
       // Column offset in the per-lane offset and an immediate scalar offset of
@@ -491,7 +455,40 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
       // the compiler only guards them against the next instructions (wait
       // states) when the scalar offset is not a register. The range check
       // still drops kOffNone + kc*128 (< 2^32).
-      __builtin_amdgcn_raw_buffer_store_b128(u, ro, (seg_ok ? voff[i] : kOffNone) + (unsigned)(kc * 128), 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(u, ro, seg_ok ? voff[i] + (unsigned)(kc * 128) : kOffNone, 0, 0);
+    }
+  };
+  auto chunk = [&](int kc, u32x4(&da)[NI], u32x4(&db)[NI]) {
+    // This chunk's loads are done once at most the later chunks' loads are
+    // outstanding: loads complete in issue order, but stores (some of them
+    // wholly out of range) may complete before earlier loads, so the wait
+    // must not count on the stores issued in between.
+    __builtin_amdgcn_s_waitcnt(vmcnt_only<(kRowPrefetch - 1) * NI * (int(NA) + int(NB))>());
+    // stage the quantity of x columns 32k+4 .. 32k+35 of the band's rows
+#pragma unroll
+    for (int i = 0; i < NI; ++i) tile_x[eidx[i]] = to_vf4(row_quantity<Q>(u2f(da[i]), u2f(db[i])));
+    wave_lds_order();
+    load_chunk(min(kc + kRowPrefetch, nchunks - 1), da, db);
+    // the previous chunk's outputs leave while this chunk's inputs come in
+    store_chunk(kc - 1);
+    float4 x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = to_f4(tile_x[rslot[j]]);
+    __builtin_amdgcn_sched_barrier(0);
+    float4 out[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      ring[(j + 1) & 3] = x[j];  // x-group 8k+j+1
+      const float4 rr = ring[(j + 1) & 3], m2 = ring[(j + 2) & 3], m1 = ring[(j + 3) & 3];
+      out[j].x = step(m2.z, rr.x);
+      out[j].y = step(m2.w, rr.y);
+      out[j].z = step(m1.x, rr.z);
+      out[j].w = step(m1.y, rr.w);
+    }
+    wave_lds_order();
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) tile_k[rslot[j]] = to_vf4(out[j]);
     }
     wave_lds_order();
   };
@@ -499,17 +496,16 @@ __device__ __forceinline__ void row_scan(__amdgpu_buffer_rsrc_t ra, __amdgpu_buf
 #pragma unroll
     for (int d = 0; d < kRowPrefetch; ++d) chunk(kc + d, qa[d], qb[d]);
   }
+  store_chunk(nchunks_pad - 1);
 }
 
 template <int SIDES>
 __global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
                                              const WorkItem* __restrict__ items) {
-  // input tiles of the two images, reused with a third tile for the three
-  // terms' outputs
-  __shared__ float4 tiles[3 * kTileF4];
-  lds_vf4* tile_a = (lds_vf4*)tiles;
-  lds_vf4* tile_b = (lds_vf4*)(tiles + kTileF4);
-  lds_vf4* tile_c = (lds_vf4*)(tiles + 2 * kTileF4);
+  // the staged quantity and the three terms' outputs
+  __shared__ float4 tiles[4 * kTileF4];
+  lds_vf4* tile_x = (lds_vf4*)tiles;
+  lds_vf4* tile_o = (lds_vf4*)(tiles + kTileF4);
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int s = it.a, c = it.b / kNumQ, q = it.b % kNumQ;
@@ -523,11 +519,11 @@ __global__ __launch_bounds__(64) void k_rows(const PairDesc* __restrict__ pairs,
   const __amdgpu_buffer_rsrc_t ro =
       buffer_rsrc(P.arena + S.hrow + (long long)(c * kNumQ + q) * S.plane + off, bytes);
   switch (q) {
-    case 0: row_scan<0>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
-    case 1: row_scan<1>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
-    case 2: row_scan<2>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
-    case 3: row_scan<3>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
-    default: row_scan<4>(ra, rb, ro, S.w, S.pitch, nrows, tile_a, tile_b, tile_c); break;
+    case 0: row_scan<0>(ra, rb, ro, S.w, S.pitch, nrows, tile_x, tile_o); break;
+    case 1: row_scan<1>(ra, rb, ro, S.w, S.pitch, nrows, tile_x, tile_o); break;
+    case 2: row_scan<2>(ra, rb, ro, S.w, S.pitch, nrows, tile_x, tile_o); break;
+    case 3: row_scan<3>(ra, rb, ro, S.w, S.pitch, nrows, tile_x, tile_o); break;
+    default: row_scan<4>(ra, rb, ro, S.w, S.pitch, nrows, tile_x, tile_o); break;
   }
 }
 
