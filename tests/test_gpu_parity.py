@@ -256,8 +256,9 @@ def _f32_bits(v):
     (2, 0.0, 1024.0, "edge ratio, every |difference| in [0, 1024] x 64 partners"),
 ])
 def test_reduced_arithmetic_equals_ieee_on_the_device(s2, which, lo, hi, what):
-    """The kernels' cube root divides by reciprocal and Newton steps, and the
-    maps' f32 and f64 quotients do the same; on the device each equals its
+    """The kernels' cube root (exponent scale table, reciprocal with one
+    Newton step and a remainder correction) and the maps' f32 and f64
+    quotients (reciprocal and Newton steps); on the device each equals its
     IEEE form on the whole swept range (the host build of the cube root, with
     the IEEE division, equals the platform cbrtf on [0, 2]: tests/test_oracle.py)."""
     import ctypes

@@ -197,9 +197,10 @@ def test_cbrt_is_the_platform_cbrtf(oracle):
 
 
 def test_kernel_cbrt_equals_libm_on_every_float(tmp_path):
-    """cbrtf_ref (the GPU kernels' cube root, ssim2_math.h) restates glibc's
-    cbrtf, which cpu.rs reaches through f32::cbrt. Built here as host code and
-    compared with the host cbrtf on every float in [0, 2] (the XYB argument
+    """cbrtf_ref and cbrtf_nonneg (the GPU kernels' cube root, ssim2_math.h,
+    with the exponent folded into a scale table) restate glibc's cbrtf, which
+    cpu.rs reaches through f32::cbrt. Built here as host code and compared
+    with the host cbrtf on every float in [0, 2] (the XYB argument
     range is [0.0038, 1.0038])."""
     import os
     import subprocess

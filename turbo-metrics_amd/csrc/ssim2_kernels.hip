@@ -80,6 +80,18 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+constexpr unsigned kRsrcWord3 = 0x00020000u;    // raw buffer, 32-bit data format
+constexpr unsigned kOffNone = 0xC0000000u;      // beyond every plane (< 2^30 floats)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const float* p, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0, (int)bytes, kRsrcWord3);
+}
+
+__device__ __forceinline__ float4 u2f(u32x4 v) {
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+
 // XYB plane of (side image, channel) at scale s for the given launch sides.
 // Both: ref -> xyb[c], dis -> xyb[3+c]. Ref: ref -> cache[c]. Dis: dis -> xyb[3+c].
 __device__ __forceinline__ float* xyb_plane(const PairDesc& P, int s, int img, int c, int sides) {
@@ -92,14 +104,34 @@ __device__ __forceinline__ float* xyb_plane(const PairDesc& P, int s, int img, i
 
 // ---------------------------------------------------------------- pyramid
 
+// 2x2 mean of the previous scale (cpu.rs:545-579): source coordinates clamped
+// to its last column and row, summed (x0,y0), (x1,y0), (x0,y1), (x1,y1);
+// at(x, y) reads the previous scale at global coordinates.
+template <typename At>
+__device__ __forceinline__ float mean2x2(int gx, int gy, int pw, int ph, At at) {
+  const int x0 = min(2 * gx, pw - 1), x1 = min(2 * gx + 1, pw - 1);
+  const int y0 = min(2 * gy, ph - 1), y1 = min(2 * gy + 1, ph - 1);
+  float sum = 0.0f;
+  sum += at(x0, y0);
+  sum += at(x1, y0);
+  sum += at(x0, y1);
+  sum += at(x1, y1);
+  return sum * 0.25f;
+}
+
 template <int SIDES>
 __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict__ pairs,
                                                       const WorkItem* __restrict__ items,
                                                       InputArgs in) {
   constexpr int NI = SideTraits<SIDES>::NI;
-  constexpr int T0 = kPyrTile;
-  __shared__ float bufA[NI * 3][T0 / 4][T0 / 4 + 1];  // scales 2, 4
-  __shared__ float bufB[NI * 3][T0 / 2][T0 / 2 + 1];  // scales 1, 3, 5
+  constexpr int T0 = kPyrTile, T1 = T0 / 2, T2 = T0 / 4, T3 = T0 / 8;
+  static_assert(T1 * T1 == kThreads, "one scale-1 pixel per thread");
+  static_assert(NI * (64 + 21) <= kThreads, "scale-2 threads and the scale 3-5 threads fit the block");
+  __shared__ float lin1[NI * 3][T1][T1 + 1];  // scale 1, linear
+  __shared__ float lin2[NI * 3][T2][T2 + 1];  // scale 2
+  __shared__ float lin3[NI * 3][T3][T3 + 1];  // scale 3
+  __shared__ float lut_s[256];
+  __shared__ double cbrt_scale_s[kCbrtScales];
   const WorkItem it = items[blockIdx.x];
   const PairDesc& P = pairs[it.pair];
   const int tid = threadIdx.x;
@@ -123,25 +155,11 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
     in.kind = P.in_kind;
   }
 
-  // the sRGB table in LDS: the lookups are divergent, so constant memory would
-  // serve them as one global load each
-  // and the cube root's factor table (a divergent index too)
-  __shared__ float lut_s[256];
-  __shared__ double cbrt_factor_s[5];
-  if (in.kind == kInputSrgbU8)
-    for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut[i];
-  if (tid < 5) cbrt_factor_s[tid] = c_cbrt_factor[tid];
-  lds_barrier();
-  lds_cfloat* lut = (lds_cfloat*)lut_s;
-  const lds_cdouble* cbrt_factor = (const lds_cdouble*)cbrt_factor_s;
-
   // Scales 0 and 1 without a scale-0 tile: each thread takes the 2x2 scale-0
   // block behind its scale-1 pixel (clamped at the image edge, as the
-  // downscale clamps), converts and stores the block's in-image pixels, and
-  // forms the scale-1 mean from the same registers. sRGB u8 input: all input
-  // bytes of the block are loaded before the first is used.
-  constexpr int T1 = T0 / 2;
-  static_assert(T1 * T1 == kThreads, "one scale-1 pixel per thread");
+  // downscale clamps), converts and stores the block, and forms the scale-1
+  // mean from the same registers. sRGB u8 input: every input byte of the
+  // block is requested before the tables are staged.
   const int sx = tid % T1, sy = tid / T1;
   const bool u8 = in.kind == kInputSrgbU8;
   unsigned short blo[NI][4];  // bytes 0, 1 as loaded
@@ -159,53 +177,80 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
       }
     }
   }
+
+  // The sRGB table in LDS (the lookups are divergent, so constant memory
+  // would serve them as one global load each) and the cube root's scale table.
+  if (u8)
+    for (int i = tid; i < 256; i += kThreads) lut_s[i] = c_lut[i];
+  for (int i = tid; i < kCbrtScales; i += kThreads) cbrt_scale_s[i] = cbrt_scale(i + kCbrtExpMin);
+  lds_barrier();
+  lds_cfloat* lut = (lds_cfloat*)lut_s;
+  const lds_cdouble* cbrt_scale_t = (const lds_cdouble*)cbrt_scale_s;
+
+  // XYB of pixel `off` of a scale's planes (global stores: the row pass
+  // reads these planes next, and measured 10 % slower after buffer stores).
+  auto store_xyb = [&](int s, int img, bool inside, unsigned off, float X, float Y, float B) {
+    if (!inside) return;
+    const ScaleDesc& S = P.sc[s];
+    float* base = xyb_plane(P, s, img, 0, SIDES);
+    gbl(base)[off] = X;
+    gbl(base + S.plane)[off] = Y;
+    gbl(base + 2 * S.plane)[off] = B;
+  };
+
   float sum1[NI][3];
 #pragma unroll
   for (int im = 0; im < NI; ++im)
 #pragma unroll
     for (int c = 0; c < 3; ++c) sum1[im][c] = 0.0f;
-  // block pixel k = (x0 + (k & 1), y0 + (k >> 1)): the summation order of
-  // cpu.rs:545-579 is (x0,y0), (x1,y0), (x0,y1), (x1,y1)
+  // Block pixel k = (x0 + (k & 1), y0 + (k >> 1)): the summation order of
+  // cpu.rs:545-579 is (x0,y0), (x1,y0), (x0,y1), (x1,y1).
+  const int x0 = tx * T0 + 2 * sx;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int ux = tx * T0 + 2 * sx + (k & 1), uy = ty * T0 + 2 * sy + (k >> 1);  // unclamped
-    const int gx = min(ux, w - 1), gy = min(uy, h - 1);
-    const bool inside = ux < w && uy < h;
+  for (int row = 0; row < 2; ++row) {
+    const int uy = ty * T0 + 2 * sy + row;
+    const int gy = min(uy, h - 1);
+    const unsigned off = (unsigned)(gy * P.sc[0].pitch + x0);
 #pragma unroll
     for (int im = 0; im < NI; ++im) {
       const int img = SIDES == kSidesDis ? 1 : im;
-      float rgb[3];
-      if (u8) {
-        const unsigned lo = blo[im][k];
-        rgb[0] = lut[lo & 255u];
-        rgb[1] = lut[lo >> 8];
-        rgb[2] = lut[bhi[im][k]];
-      } else {
-        read_linear(input_of(in, img), gx, gy, lut, rgb);
-      }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) sum1[im][c] += rgb[c];
-      if (!inside) continue;
-      if (in.kind != kInputLinearF32) {
-        float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
-        if (lo) {
-          const long long lp = img == 0 ? in.lin_ref_pitch : in.lin_dis_pitch;
-          gfloat* q = gbl(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(lo) + gy * lp)) + gx * 3;
-          q[0] = rgb[0];
-          q[1] = rgb[1];
-          q[2] = rgb[2];
+      for (int kx = 0; kx < 2; ++kx) {
+        const int k = 2 * row + kx;
+        const int ux = x0 + kx;
+        const int gx = min(ux, w - 1);
+        float rgb[3];
+        if (u8) {
+          const unsigned lo = blo[im][k];
+          rgb[0] = lut[lo & 255u];
+          rgb[1] = lut[lo >> 8];
+          rgb[2] = lut[bhi[im][k]];
+        } else {
+          read_linear(input_of(in, img), gx, gy, lut, rgb);
         }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sum1[im][c] += rgb[c];
+        if (in.kind != kInputLinearF32 && ux < w && uy < h) {
+          float* lo = img == 0 ? in.lin_ref_out : in.lin_dis_out;
+          if (lo) {
+            const long long lp = img == 0 ? in.lin_ref_pitch : in.lin_dis_pitch;
+            gfloat* q = gbl(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(lo) + gy * lp)) + gx * 3;
+            q[0] = rgb[0];
+            q[1] = rgb[1];
+            q[2] = rgb[2];
+          }
+        }
+        float X, Y, B;
+        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_scale_t);
+        store_xyb(0, img, ux < w && uy < h, off + kx, X, Y, B);
       }
-      float X, Y, B;
-      linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_factor);
-      const long long off = (long long)gy * P.sc[0].pitch + gx;
-      gbl(xyb_plane(P, 0, img, 0, SIDES))[off] = X;
-      gbl(xyb_plane(P, 0, img, 1, SIDES))[off] = Y;
-      gbl(xyb_plane(P, 0, img, 2, SIDES))[off] = B;
     }
   }
-  if (P.nscales > 1) {
+  if (P.nscales < 2) return;
+  {
     const int gx = tx * T1 + sx, gy = ty * T1 + sy;
+    const bool inside = gx < P.sc[1].w && gy < P.sc[1].h;
+    const unsigned off = (unsigned)(gy * P.sc[1].pitch + gx);
 #pragma unroll
     for (int im = 0; im < NI; ++im) {
       const int img = SIDES == kSidesDis ? 1 : im;
@@ -213,65 +258,68 @@ __global__ __launch_bounds__(kThreads) void k_pyramid(const PairDesc* __restrict
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         rgb[c] = sum1[im][c] * 0.25f;
-        bufB[im * 3 + c][sy][sx] = rgb[c];
+        lin1[im * 3 + c][sy][sx] = rgb[c];
       }
-      if (gx < P.sc[1].w && gy < P.sc[1].h) {
-        float X, Y, B;
-        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_factor);
-        const long long off = (long long)gy * P.sc[1].pitch + gx;
-        gbl(xyb_plane(P, 1, img, 0, SIDES))[off] = X;
-        gbl(xyb_plane(P, 1, img, 1, SIDES))[off] = Y;
-        gbl(xyb_plane(P, 1, img, 2, SIDES))[off] = B;
-      }
+      float X, Y, B;
+      linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_scale_t);
+      store_xyb(1, img, inside, off, X, Y, B);
     }
   }
+  if (P.nscales < 3) return;
 
-  // Scales 2..n-1: 2x2 mean of the previous scale (cpu.rs:545-579), then XYB.
-  for (int s = 2; s < P.nscales; ++s) {
-    lds_barrier();
-    const int T = T0 >> s;
-    const int pw = P.sc[s - 1].w, ph = P.sc[s - 1].h;
-    const int cw = P.sc[s].w, ch = P.sc[s].h;
-    const int ox0 = tx * T, oy0 = ty * T;
-    for (int p = tid; p < T * T; p += kThreads) {
-      const int lx = p % T, ly = p / T;
-      const int gx = ox0 + lx, gy = oy0 + ly;
-      if (gx >= cw || gy >= ch) continue;
-      int xs[2], ys[2];
-      for (int k = 0; k < 2; ++k) {
-        xs[k] = min(2 * gx + k, pw - 1) - 2 * ox0;
-        ys[k] = min(2 * gy + k, ph - 1) - 2 * oy0;
-      }
-      for (int im = 0; im < NI; ++im) {
-        const int img = SIDES == kSidesDis ? 1 : im;
-        float rgb[3];
-        for (int c = 0; c < 3; ++c) {
-          float sum = 0.0f;
-          if (s & 1) {
-            sum += bufA[im * 3 + c][ys[0]][xs[0]];
-            sum += bufA[im * 3 + c][ys[0]][xs[1]];
-            sum += bufA[im * 3 + c][ys[1]][xs[0]];
-            sum += bufA[im * 3 + c][ys[1]][xs[1]];
-          } else {
-            sum += bufB[im * 3 + c][ys[0]][xs[0]];
-            sum += bufB[im * 3 + c][ys[0]][xs[1]];
-            sum += bufB[im * 3 + c][ys[1]][xs[0]];
-            sum += bufB[im * 3 + c][ys[1]][xs[1]];
-          }
-          rgb[c] = sum * 0.25f;
-          if (s & 1)
-            bufB[im * 3 + c][ly][lx] = rgb[c];
-          else
-            bufA[im * 3 + c][ly][lx] = rgb[c];
-        }
-        float X, Y, B;
-        linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_factor);
-        const long long off = (long long)gy * P.sc[s].pitch + gx;
-        gbl(xyb_plane(P, s, img, 0, SIDES))[off] = X;
-        gbl(xyb_plane(P, s, img, 1, SIDES))[off] = Y;
-        gbl(xyb_plane(P, s, img, 2, SIDES))[off] = B;
-      }
+  // Scales 2..5 (8x8, 4x4, 2x2 and 1x1 pixels of the tile): the means level
+  // by level with a barrier each, then every pixel's XYB at once. Threads
+  // 0 .. 64 NI - 1 take a scale-2 pixel of one image; the next 21 NI threads
+  // a scale-3, -4 or -5 pixel (16 + 4 + 1 per image).
+  lds_barrier();
+  const int ox1 = tx * T1, oy1 = ty * T1, ox2 = tx * T2, oy2 = ty * T2, ox3 = tx * T3, oy3 = ty * T3;
+  const bool s2_thread = tid < 64 * NI;
+  const int t = tid - 64 * NI;
+  const bool tail_thread = t >= 0 && t < 21 * NI;
+  const int im = s2_thread ? tid / 64 : (tail_thread ? t / 21 : 0);
+  const int j = tail_thread ? t % 21 : 0;
+  const int s = s2_thread ? 2 : (j < 16 ? 3 : (j < 20 ? 4 : 5));
+  const int T = T0 >> s;
+  const int li = s2_thread ? tid % 64 : (j < 16 ? j : (j < 20 ? j - 16 : 0));
+  const int gx = tx * T + li % T, gy = ty * T + li / T;
+  // a pixel of a scale the image has (the scale descriptors past the last
+  // one are not read)
+  const bool live = (s2_thread || tail_thread) && s < P.nscales;
+  float rgb[3] = {0.0f, 0.0f, 0.0f};
+  if (s2_thread) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rgb[c] = mean2x2(gx, gy, P.sc[1].w, P.sc[1].h,
+                       [&](int x, int y) { return lin1[im * 3 + c][y - oy1][x - ox1]; });
+      lin2[im * 3 + c][gy - oy2][gx - ox2] = rgb[c];
     }
+  }
+  lds_barrier();
+  if (live && s == 3) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rgb[c] = mean2x2(gx, gy, P.sc[2].w, P.sc[2].h,
+                       [&](int x, int y) { return lin2[im * 3 + c][y - oy2][x - ox2]; });
+      lin3[im * 3 + c][gy - oy3][gx - ox3] = rgb[c];
+    }
+  }
+  lds_barrier();
+  if (live && s >= 4) {
+    const int w3 = P.sc[3].w, h3 = P.sc[3].h, w4 = P.sc[4].w, h4 = P.sc[4].h;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      auto at3 = [&](int x, int y) { return lin3[im * 3 + c][y - oy3][x - ox3]; };
+      if (s == 4)
+        rgb[c] = mean2x2(gx, gy, w3, h3, at3);
+      else
+        rgb[c] = mean2x2(gx, gy, w4, h4, [&](int x, int y) { return mean2x2(x, y, w3, h3, at3); });
+    }
+  }
+  if (live) {
+    const int img = SIDES == kSidesDis ? 1 : im;
+    float X, Y, B;
+    linear_to_xyb(rgb[0], rgb[1], rgb[2], X, Y, B, cbrt_scale_t);
+    store_xyb(s, img, gx < P.sc[s].w && gy < P.sc[s].h, (unsigned)(gy * P.sc[s].pitch + gx), X, Y, B);
   }
 }
 
@@ -304,17 +352,6 @@ constexpr int kRowPrefetch = 2;                 // chunks of row-pass input in f
 constexpr int kTileRowF4 = kChunk / 4;
 constexpr int kTileF4 = kRowsPerWave * kTileRowF4 + 3;
 __device__ __forceinline__ int tile_slot(int r, int g) { return r * kTileRowF4 + (g ^ (r & 7)); }
-typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
-constexpr unsigned kRsrcWord3 = 0x00020000u;    // raw buffer, 32-bit data format
-constexpr unsigned kOffNone = 0xC0000000u;      // beyond every plane (< 2^30 floats)
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const float* p, long long bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0, (int)bytes, kRsrcWord3);
-}
-
-__device__ __forceinline__ float4 u2f(u32x4 v) {
-  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
-}
 
 // The row pass exchanges data between the lanes of one wave through LDS. A
 // wave's LDS operations execute in issue order, so only the compiler's
@@ -844,23 +881,28 @@ __constant__ float c_check_partners[64] = {
 
 // Diagnostics of the reduced arithmetic against the IEEE forms, on the
 // device, for every float x whose bits lie in [lo, hi]; differing results are
-// counted in *mismatches. which 0: the cube root of x. which 1: the SSIM
+// counted in *mismatches. which 0: the kernels' cube root of x >= 0 (scale
+// table, reciprocal division) against the reference form with the IEEE
+// division, which equals the platform cbrtf on [0, 2]. which 1: the SSIM
 // map's 1 - q for q = a / x in f32, a over the partners (x is the
 // denominator, >= C2 minus rounding in the maps). which 2: the edge ratio with
 // |dis - mu2| = x and |src - mu1| over the partners' magnitudes.
 __global__ __launch_bounds__(kThreads) void k_check_arithmetic(int which, unsigned lo, unsigned hi,
                                                                unsigned long long* __restrict__ mismatches) {
   __shared__ double factor_s[5];
+  __shared__ double scale_s[kCbrtScales];
   if (threadIdx.x < 5) factor_s[threadIdx.x] = c_cbrt_factor[threadIdx.x];
+  for (int i = threadIdx.x; i < kCbrtScales; i += kThreads) scale_s[i] = cbrt_scale(i + kCbrtExpMin);
   lds_barrier();
   const lds_cdouble* factor = (const lds_cdouble*)factor_s;
+  const lds_cdouble* scale = (const lds_cdouble*)scale_s;
   unsigned long long bad = 0;
   const unsigned long long n = (unsigned long long)hi - lo + 1;
   for (unsigned long long i = (unsigned long long)blockIdx.x * kThreads + threadIdx.x; i < n;
        i += (unsigned long long)gridDim.x * kThreads) {
     const float x = __uint_as_float(lo + (unsigned)i);
     if (which == 0) {
-      bad += __float_as_uint(cbrtf_ref<false>(x, factor)) != __float_as_uint(cbrtf_ref<true>(x, factor));
+      bad += __float_as_uint(cbrtf_nonneg<false>(x, scale)) != __float_as_uint(cbrtf_ref<true>(x, factor));
     } else if (which == 1) {
       for (int k = 0; k < 64; ++k) {
         const float a = c_check_partners[k];
@@ -911,20 +953,20 @@ __global__ __launch_bounds__(kThreads) void k_downscale_by_2(const float* __rest
 
 // Kernel::linear_to_xyb (crates/ssimulacra2-cuda-kernel/src/xyb.rs:81-102,
 // cpu.rs:458-496): packed f32 linear RGB to packed positive XYB, one thread
-// per pixel, the pyramid's arithmetic (cube root with its factor table in LDS).
+// per pixel, the pyramid's arithmetic (cube root with its scale table in LDS).
 __global__ __launch_bounds__(kThreads) void k_linear_to_xyb(const float* __restrict__ src, long long src_pitch,
                                                             float* __restrict__ dst, long long dst_pitch, int w,
                                                             int h) {
-  __shared__ double factor_s[5];
-  if (threadIdx.x < 5) factor_s[threadIdx.x] = c_cbrt_factor[threadIdx.x];
+  __shared__ double scale_s[kCbrtScales];
+  for (int i = threadIdx.x; i < kCbrtScales; i += kThreads) scale_s[i] = cbrt_scale(i + kCbrtExpMin);
   lds_barrier();
-  const lds_cdouble* factor = (const lds_cdouble*)factor_s;
+  const lds_cdouble* scale = (const lds_cdouble*)scale_s;
   const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= w || y >= h) return;
   gcfloat* in = gbl(const_cast<float*>(reinterpret_cast<const float*>(reinterpret_cast<const unsigned char*>(src) + y * src_pitch))) + 3 * x;
   gfloat* o = gbl(reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(dst) + y * dst_pitch)) + 3 * x;
   float X, Y, B;
-  linear_to_xyb(in[0], in[1], in[2], X, Y, B, factor);
+  linear_to_xyb(in[0], in[1], in[2], X, Y, B, scale);
   o[0] = X;
   o[1] = Y;
   o[2] = B;

@@ -100,16 +100,67 @@ SSIM2_HD float cbrtf_ref(float x) {
   return cbrtf_ref(x, factor);
 }
 
+// The same cube root for x >= 0 with the exponent handling folded into one
+// table: for x = m * 2^e, cbrtf_ref rounds q * factor[e % 3 + 2] to float and
+// scales by 2^(e / 3); scaling by a power of two commutes with both roundings
+// (the double product stays normal, the float result is at least
+// cbrt(2^-149) > 2^-50), so y = (float)(q * scale[e]) with
+// scale[e] = factor[e % 3 + 2] * 2^(e / 3) is the same float. Exponents of
+// positive finite floats run from -148 to 128.
+constexpr int kCbrtExpMin = -148;
+constexpr int kCbrtScales = 129 - kCbrtExpMin;  // e = -148 .. 128
+
+SSIM2_HD double cbrt_scale(int e) {
+  const int r = e % 3, k = e / 3;  // C remainder and quotient, as cbrtf_ref
+  const double f = r == -2   ? 1.0 / 1.5874010519681994748
+                   : r == -1 ? 1.0 / 1.2599210498948731648
+                   : r == 0  ? 1.0
+                   : r == 1  ? 1.2599210498948731648
+                             : 1.5874010519681994748;
+  return ldexp(f, k);
+}
+
+// `scale` indexes like cbrt_scale(e) at e - kCbrtExpMin (an array in any
+// address space: the kernels keep it in LDS). On the device the quotient is
+// the hardware reciprocal with one Newton step and one remainder correction;
+// its equality to the IEEE form for every float in [0, 2] is checked on the
+// device. Zero and +infinity return themselves (x + x in cbrtf_ref).
+template <bool kIeeeDiv = false, typename Scales>
+SSIM2_HD float cbrtf_nonneg(float x, const Scales& scale) {
+  int e;
+  const float m = frexpf(x, &e);
+  const double md = m;
+  const float u = (float)(0.492659620528969547 + (0.697570460207922770 - 0.191502161678719066 * md) * md);
+  const float t2 = u * u * u;
+  const double ud = u, t2d = t2;
+  const double num = ud * fma(2.0, md, t2d), den = fma(2.0, t2d, md);
+  double q;
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (!kIeeeDiv) {
+    double r = __builtin_amdgcn_rcp(den);
+    r = fma(r, fma(-den, r, 1.0), r);
+    q = num * r;
+    q = fma(fma(-den, q, num), r, q);
+  } else {
+    q = num / den;
+  }
+#else
+  q = num / den;
+#endif
+  const float y = (float)(q * scale[e - kCbrtExpMin]);
+  return (x == 0.0f || isinf(x)) ? x : y;
+}
+
 // cpu.rs:458-496 px_linear_rgb_to_xyb followed by the positive shift;
-// `factor` is the cube root's factor table (SSIM2_CBRT_FACTORS_INIT).
-template <typename Factors>
-SSIM2_HD void linear_to_xyb(float r, float g, float b, float& X, float& Y, float& B, const Factors& factor) {
+// `scale` is the cube root's table (cbrt_scale).
+template <typename Scales>
+SSIM2_HD void linear_to_xyb(float r, float g, float b, float& X, float& Y, float& B, const Scales& scale) {
   float rg = fmaf(kM00, r, fmaf(kM01, g, fmaf(kM02, b, kB0)));
   float gr = fmaf(kM10, r, fmaf(kM11, g, fmaf(kM12, b, kB0)));
   float bb = fmaf(kM20, r, fmaf(kM21, g, fmaf(kM22, b, kB0)));
-  rg = cbrtf_ref(fmaxf(rg, 0.0f), factor) - kB0Root;
-  gr = cbrtf_ref(fmaxf(gr, 0.0f), factor) - kB0Root;
-  bb = cbrtf_ref(fmaxf(bb, 0.0f), factor) - kB0Root;
+  rg = cbrtf_nonneg(fmaxf(rg, 0.0f), scale) - kB0Root;
+  gr = cbrtf_nonneg(fmaxf(gr, 0.0f), scale) - kB0Root;
+  bb = cbrtf_nonneg(fmaxf(bb, 0.0f), scale) - kB0Root;
   float x = 0.5f * (rg - gr);
   float y = 0.5f * (rg + gr);
   X = fmaf(x, 14.0f, 0.42f);
