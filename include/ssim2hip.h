@@ -283,10 +283,14 @@ ssim2_status ssim2_batch_compute_sync(ssim2_batch* batch, const ssim2_pair* pair
 ssim2_status ssim2_batch_query(ssim2_batch* batch, int* done);
 /* Per-scale sums of pair `index` of the last batch (layout of ssim2_get_sums). */
 ssim2_status ssim2_batch_get_sums(ssim2_batch* batch, uint32_t index, double* sums108, int* dims12);
-/* Output tile shape (0: 32x16, 1: 64x16, 2: 32x32, 3: 32x24) and grid size in
- * multiples of the CU count (0: one block per tile, sized per batch);
- * ssim2_batch_autotune times every combination on the given pairs and keeps
- * the fastest. `report` (optional) receives a table. */
+/* Output tile shape (0: 32x16, 1: 64x16, 2: 32x32, 3: 32x24, 4: 64x32) and
+ * grid size in multiples of the CU count (0: one block per tile, sized per
+ * batch); ssim2_batch_autotune times every combination on the given pairs and
+ * keeps the fastest. With binary16 storage, shape 4 runs the blur on the
+ * matrix cores (binary16 operands split into hi + lo parts, f32 sums; scores
+ * within about 1e-3 of the other shapes) and assumes linear inputs in [0, 1]
+ * as sRGB and YUV inputs always are: its blurred squares stay finite for
+ * shifted XYB values up to about 11. `report` (optional) receives a table. */
 ssim2_status ssim2_batch_set_tiling(ssim2_batch* batch, uint32_t tile_cfg, uint32_t grid_mult);
 ssim2_status ssim2_batch_get_tiling(const ssim2_batch* batch, uint32_t* tile_cfg, uint32_t* grid_mult);
 ssim2_status ssim2_batch_autotune(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,

@@ -671,7 +671,7 @@ ssim2_status ssim2_batch_compute_sync(ssim2_batch* b, const ssim2_pair* pairs, u
 
 ssim2_status ssim2_batch_set_tiling(ssim2_batch* b, uint32_t tile_cfg, uint32_t grid_mult) {
   if (!b || tile_cfg >= (uint32_t)kNumTileCfgs || grid_mult > 1024)
-    return bfail(SSIM2_INVALID_ARGUMENT, "tile_cfg in [0, 3], grid_mult in [0, 1024]");
+    return bfail(SSIM2_INVALID_ARGUMENT, "tile_cfg in [0, 4], grid_mult in [0, 1024]");
   b->tile_cfg = (int)tile_cfg;
   b->grid_mult = (int)grid_mult;
   return SSIM2_OK;

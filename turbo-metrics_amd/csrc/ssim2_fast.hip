@@ -686,6 +686,343 @@ void k_fast_tile(const PairDesc* __restrict__ pairs,
   }
 }
 
+// ---------------------------------------------------------------- fused tile on the matrix cores
+
+// The separable 9-tap FIR of one 16-column strip is a banded matrix product,
+// so the f16-storage form can run both passes on the matrix cores
+// (v_mfma_f32_16x16x32_f16: products of binary16 operands are exact, sums in
+// f32). A wave owns 16 output columns x 32 output rows of one channel; a
+// block is four such waves side by side (a 64x32 tile). Lane (g, l) below
+// means lane 16 g + l.
+//
+// Horizontal pass, for each of three row blocks b (input rows
+// y0 - 4 + 16 b + l): A = 16 rows x 32 input columns (lane (g, l) holds the 8
+// columns xw - 8 + 8 g .. + 7 of row l, one aligned 16-byte load; the outer 4
+// columns on each side only widen the aligned window and carry weight 0),
+// B = the band matrix W[k][n] = tap(|k - 8 - n|), D = 16 rows x 16 output
+// columns, lane (g, n) holding rows 4 g .. 4 g + 3 of column n.
+// Vertical pass, transposed so that a lane ends with four neighbouring
+// columns of one output row: D' = T^T x Wv, where A = T^T takes row blocks b
+// and b + 1 of the horizontal results as they sit in the registers (lane
+// (g, n): elements 0-3 = block b rows 4 g + i, elements 4-7 = block b + 1
+// rows 4 g + i; A and B share the K order, so any order of the T rows works
+// as long as the weights follow it) and Wv[k][r] = tap(|rho(k) - r - 4|).
+// D' lane (g, r) holds output row r, columns 4 g .. 4 g + 3.
+// Precision: binary16 operands carry 11 significant bits, too few for the
+// blurred squares (their f16 rounding alone moves scores by 0.3), so every
+// operand that is not already a binary16 value is split: taps as hi + lo
+// (hi = f16(tap), lo = f16(tap - hi)); r'^2, d'^2 and r'd' of the stored
+// binary16 planes as hi + lo exactly (hi = f16(x y); lo = fma(x, y, -hi) in
+// binary16 is the exact remainder), so blur(r'^2 + d'^2) is the sum of
+// hi.Whi + hi.Wlo + lo.Whi over both squares; the horizontal results T as
+// hi + lo likewise before the vertical pass. The lo.lo products (below
+// 2^-22 relative) are dropped.
+typedef _Float16 mx_h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 mx_h4 __attribute__((ext_vector_type(4)));
+typedef float mx_f4 __attribute__((ext_vector_type(4)));
+constexpr int kMxCols = 16, kMxRows = 32, kMxWaves = 4;
+static_assert(kMxWaves * kMxCols == 64 && kMxRows == 32, "the 64x32 tile of tile_cfg 4");
+
+__device__ __forceinline__ mx_f4 mx_mfma(mx_h8 a, mx_h8 b, mx_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// Power-of-two operand scales (exact in binary16 and f32) that keep the lo
+// parts of products, taps and horizontal results out of the binary16
+// subnormal range, where they would lose bits: the planes x4 before they are
+// squared, the taps x16 (x4 for the horizontal pass of the means). The
+// vertical results are then 64 times the blurred means and 4096 = 64^2 times
+// the blurred squares, so the statistics come out in one common scale: var
+// and cov x4096, the means x64. The squares stay finite in binary16 for
+// |shifted XYB| up to about 11 (linear inputs in [0, 1] give at most 0.45).
+constexpr float kMxTapScale = 16.0f, kMxMeanTapScale = 4.0f;
+constexpr float kMxMeanScale = 64.0f, kMxVarScale = kMxMeanScale * kMxMeanScale;
+
+// Tap |t| of the FIR (0 beyond 4) times `scale`, split hi/lo in binary16.
+__device__ __forceinline__ void mx_tap(int t, float scale, _Float16& hi, _Float16& lo) {
+  t = t < 0 ? -t : t;
+  const float v = scale * (t == 0 ? kH0 : t == 1 ? kH1 : t == 2 ? kH2 : t == 3 ? kH3 : t == 4 ? kH4 : 0.0f);
+  hi = (_Float16)v;
+  lo = (_Float16)(v - (float)hi);
+}
+
+// Constant operands of a lane: horizontal band W (lane (g, n = l): k = 8 g + j)
+// and vertical band Wv (lane (g, r = l): rho = 4 g + j, or 16 + 4 g + j - 4).
+struct MxTaps {
+  mx_h8 h_hi, h_lo, m_hi, m_lo, v_hi, v_lo;  // horizontal (squares), horizontal (means), vertical
+};
+
+// One channel's inputs of a wave: the horizontal A operands of row blocks
+// 0..2 for both images, and the pixels of the wave's maps (output row
+// y0 + 16 ob + l, columns xw + 4 g .. + 3).
+struct MxIn {
+  mx_h8 a[3], d[3];
+  mx_h4 src[2], dis[2];
+};
+
+// Tile geometry of a wave: its first output column xw, the tile's first row
+// y0, the scale's size and pitch.
+struct MxGeo {
+  int xw, y0, w, h, g, l;
+  long long pitch;
+};
+
+// 8 binary16 values of row yr, columns xc .. xc + 7 (xc a multiple of 8),
+// zero outside the image. kEdge = false: the whole window lies inside the
+// image (the caller checked).
+template <bool kEdge>
+__device__ __forceinline__ mx_h8 mx_load8(const gT<_Float16>* plane, const MxGeo& G, int yr, int xc, bool used) {
+  mx_h8 v = {};
+  if (!kEdge) {
+    if (used) v = *(const gT<mx_h8>*)(plane + (long long)yr * G.pitch + xc);
+    return v;
+  }
+  // a window starting inside the row ends inside its pitch (a multiple of 16)
+  if (used && yr >= 0 && yr < G.h && xc >= 0 && xc < G.w) {
+    v = *(const gT<mx_h8>*)(plane + (long long)yr * G.pitch + xc);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (xc + j >= G.w) v[j] = (_Float16)0.0f;
+  }
+  return v;
+}
+
+template <bool kEdge>
+__device__ __forceinline__ void mx_load(MxIn& in, const gT<_Float16>* pr, const gT<_Float16>* pd, const MxGeo& G) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int yr = G.y0 - 4 + 16 * b + G.l, xc = G.xw - 8 + 8 * G.g;
+    const bool used = b < 2 || G.l < 8;  // rows past y0 + 35 carry weight 0
+    in.a[b] = mx_load8<kEdge>(pr, G, yr, xc, used);
+    in.d[b] = mx_load8<kEdge>(pd, G, yr, xc, used);
+  }
+#pragma unroll
+  for (int ob = 0; ob < 2; ++ob) {
+    const int gy = G.y0 + 16 * ob + G.l, gx0 = G.xw + 4 * G.g;
+    in.src[ob] = mx_h4{};
+    in.dis[ob] = mx_h4{};
+    // a 4-column group starting inside the row ends inside its pitch
+    if (!kEdge || (gy < G.h && gx0 < G.w)) {
+      in.src[ob] = *(const gT<mx_h4>*)(pr + (long long)gy * G.pitch + gx0);
+      in.dis[ob] = *(const gT<mx_h4>*)(pd + (long long)gy * G.pitch + gx0);
+    }
+  }
+}
+
+// Two f32 values rounded to binary16, packed (one v_cvt_pk_f16_f32).
+__device__ __forceinline__ unsigned mx_pk(float x, float y) {
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  const h2 v = {(_Float16)x, (_Float16)y};
+  return __builtin_bit_cast(unsigned, v);
+}
+
+// The remainders x - hi.lo and y - hi.hi in f32 (exact: hi is x, y rounded
+// to binary16), one mixed-precision fma each with the binary16 operand read
+// in place; packed to binary16 again.
+__device__ __forceinline__ unsigned mx_rem(unsigned hi, float x, float y) {
+  float rx, ry;
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(rx) : "v"(hi), "v"(x));
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(ry) : "v"(hi), "v"(y));
+  return mx_pk(rx, ry);
+}
+
+// hi and lo binary16 parts of the f32 values u[0..3], v[0..3] as one operand.
+__device__ __forceinline__ void mx_split(mx_f4 u, mx_f4 v, mx_h8& hi, mx_h8& lo) {
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  u4 h, l;
+  h[0] = mx_pk(u[0], u[1]);
+  h[1] = mx_pk(u[2], u[3]);
+  h[2] = mx_pk(v[0], v[1]);
+  h[3] = mx_pk(v[2], v[3]);
+  l[0] = mx_rem(h[0], u[0], u[1]);
+  l[1] = mx_rem(h[1], u[2], u[3]);
+  l[2] = mx_rem(h[2], v[0], v[1]);
+  l[3] = mx_rem(h[3], v[2], v[3]);
+  hi = __builtin_bit_cast(mx_h8, h);
+  lo = __builtin_bit_cast(mx_h8, l);
+}
+
+// Horizontal pass of row block b: T of r', d', Q = r'^2 + d'^2 - 2 r'd', r'd'.
+__device__ __forceinline__ void mx_hpass(const mx_h8& a, const mx_h8& d, const MxTaps& K, mx_f4 (&H)[4]) {
+  const mx_f4 z4 = {0.0f, 0.0f, 0.0f, 0.0f};
+  const mx_h8 four = {4.0f16, 4.0f16, 4.0f16, 4.0f16, 4.0f16, 4.0f16, 4.0f16, 4.0f16};
+  const mx_h8 a4 = a * four, d4 = d * four;
+  const mx_h8 a2h = a4 * a4, d2h = d4 * d4, adh = a4 * d4;
+  const mx_h8 a2l = __builtin_elementwise_fma(a4, a4, -a2h);
+  const mx_h8 d2l = __builtin_elementwise_fma(d4, d4, -d2h);
+  const mx_h8 adl = __builtin_elementwise_fma(a4, d4, -adh);
+  H[0] = mx_mfma(a, K.m_hi, mx_mfma(a, K.m_lo, z4));
+  H[1] = mx_mfma(d, K.m_hi, mx_mfma(d, K.m_lo, z4));
+  // r'^2 and d'^2 in chains of their own, in the order of r'd''s, and kept as
+  // Q = (r'^2 + d'^2) - 2 r'd' (blurred): for identical images both chains
+  // equal r'd''s bit for bit, Q is exactly 0, and the vertical pass rebuilds
+  // blur(r'^2 + d'^2) as exactly twice blur(r'd'), so var1 + var2 is exactly
+  // 2 cov (the quotient 1, the score 100)
+  const mx_f4 sa = mx_mfma(a2h, K.h_hi, mx_mfma(a2h, K.h_lo, mx_mfma(a2l, K.h_hi, z4)));
+  const mx_f4 sd = mx_mfma(d2h, K.h_hi, mx_mfma(d2h, K.h_lo, mx_mfma(d2l, K.h_hi, z4)));
+  H[3] = mx_mfma(adh, K.h_hi, mx_mfma(adh, K.h_lo, mx_mfma(adl, K.h_hi, z4)));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) H[2][i] = fmaf(-2.0f, H[3][i], sa[i] + sd[i]);
+}
+
+// Vertical pass of one quantity from row blocks b (upper) and b + 1 (lower).
+__device__ __forceinline__ mx_f4 mx_vpass(mx_f4 upper, mx_f4 lower, const MxTaps& K) {
+  const mx_f4 z4 = {0.0f, 0.0f, 0.0f, 0.0f};
+  mx_h8 thi, tlo;
+  mx_split(upper, lower, thi, tlo);
+  return mx_mfma(thi, K.v_hi, mx_mfma(thi, K.v_lo, mx_mfma(tlo, K.v_hi, z4)));
+}
+
+// Error maps of output block ob (row y0 + 16 ob + l, columns xw + 4 g + i)
+// from the blurred quantities V and the pixels; the same formulas as
+// k_fast_tile's (zero-padding terms near the border).
+template <bool kEdge>
+__device__ __forceinline__ void mx_maps(const mx_f4 (&V)[4], mx_h4 src, mx_h4 dis, float sh, const MxGeo& G, int ob,
+                                        float (&a6)[6]) {
+  const int gy = G.y0 + 16 * ob + G.l, gx0 = G.xw + 4 * G.g;
+  const float b1y = kEdge ? tap_sum(gy, G.h) : 1.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gx = gx0 + i;
+    // all statistics in the common scale: means x64, var and cov x4096
+    const float m1 = V[0][i], m2 = V[1][i], s12 = V[3][i], ssum = fmaf(2.0f, s12, V[2][i]);
+    float vsum = ssum - (m1 * m1 + m2 * m2), cov = s12 - m1 * m2;
+    float ec = 0.0f;
+    if constexpr (kEdge) {
+      if (gx >= G.w || gy >= G.h) continue;
+      const float b1 = tap_sum(gx, G.w) * b1y;
+      ec = (1.0f - b1) * (kMxMeanScale * sh);
+      const float t = (m1 + m2) + (kMxMeanScale * sh) * b1;
+      vsum = fmaf(2.0f * ec, t, vsum);
+      cov = fmaf(ec, t, cov);
+    }
+    const float md = m1 - m2;
+    const float num_m = fmaf(md, md * (-1.0f / kMxVarScale), 1.0f);
+    const float num_s = fmaf(2.0f, cov, kMxVarScale * kC2);
+    const float den = vsum + kMxVarScale * kC2;
+    const float nm = num_m * num_s;
+    const float dd = fmaxf(1.0f - (nm == den ? 1.0f : nm * __builtin_amdgcn_rcpf(den)), 0.0f);
+    const float d2 = dd * dd;
+    a6[0] += dd;
+    a6[1] = fmaf(d2, d2, a6[1]);
+    // x - mu scaled by 64, and 1 + |x - mu| = fma(|64 (x - mu)|, 1/64, 1)
+    float rn = fmaf((float)dis[i], kMxMeanScale, -m2), rd = fmaf((float)src[i], kMxMeanScale, -m1);
+    if constexpr (kEdge) {
+      rn += ec;
+      rd += ec;
+    }
+    const float en = fmaf(fabsf(rn), 1.0f / kMxMeanScale, 1.0f), ed = fmaf(fabsf(rd), 1.0f / kMxMeanScale, 1.0f);
+    const float d1 = en == ed ? 0.0f : fmaf(en, __builtin_amdgcn_rcpf(ed), -1.0f);
+    const float art = fmaxf(d1, 0.0f), det = fmaxf(-d1, 0.0f);
+    const float a2 = art * art, e2 = det * det;
+    a6[2] += art;
+    a6[3] = fmaf(a2, a2, a6[3]);
+    a6[4] += det;
+    a6[5] = fmaf(e2, e2, a6[5]);
+  }
+}
+
+// One channel of a wave: row blocks 0 and 1, output block 0, then row block
+// 2 (in block 0's registers) and output block 1.
+template <bool kEdge>
+__device__ __forceinline__ void mx_channel(const MxIn& in, const MxTaps& K, float sh, const MxGeo& G,
+                                           float (&a6)[6]) {
+  mx_f4 H0[4], H1[4], V[4];
+  mx_hpass(in.a[0], in.d[0], K, H0);
+  mx_hpass(in.a[1], in.d[1], K, H1);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) V[q] = mx_vpass(H0[q], H1[q], K);
+  mx_maps<kEdge>(V, in.src[0], in.dis[0], sh, G, 0, a6);
+  mx_hpass(in.a[2], in.d[2], K, H0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) V[q] = mx_vpass(H1[q], H0[q], K);
+  mx_maps<kEdge>(V, in.src[1], in.dis[1], sh, G, 1, a6);
+}
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_fast_tile_mfma(const PairDesc* __restrict__ pairs,
+                                                             const int* __restrict__ prefix,
+                                                             const int* __restrict__ meta) {
+  static_assert(kThreads == 64 * kMxWaves, "four waves per block");
+  __shared__ double red[kMxWaves][18];
+  const int npairs = meta[0], n = prefix[npairs];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  MxGeo G;
+  G.g = lane >> 4;
+  G.l = lane & 15;
+  MxTaps K;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    _Float16 hi, lo;
+    mx_tap(8 * G.g + j - 8 - G.l, kMxTapScale, hi, lo);
+    K.h_hi[j] = hi;
+    K.h_lo[j] = lo;
+    mx_tap(8 * G.g + j - 8 - G.l, kMxMeanTapScale, hi, lo);
+    K.m_hi[j] = hi;
+    K.m_lo[j] = lo;
+    const int rho = j < 4 ? 4 * G.g + j : 16 + 4 * G.g + (j - 4);
+    mx_tap(rho - G.l - 4, kMxTapScale, hi, lo);
+    K.v_hi[j] = hi;
+    K.v_lo[j] = lo;
+  }
+  ItemRange items(prefix, npairs, n);
+  for (int idx = items.first; idx < items.end; ++idx) {
+    const int pi = items.pair_of(idx);
+    const PairDesc& P = pairs[pi];
+    int s = 0, tile = idx - prefix[pi];
+    while (s + 1 < P.nscales && tile >= P.sc[s + 1].tile0) ++s;
+    tile -= P.sc[s].tile0;
+    const ScaleDesc& S = P.sc[s];
+    G.w = S.w;
+    G.h = S.h;
+    G.pitch = S.pitch;
+    const int x0 = (tile % S.tiles_x) * (kMxWaves * kMxCols);
+    G.y0 = (tile / S.tiles_x) * kMxRows;
+    G.xw = x0 + kMxCols * wv;
+    // every input window of the tile inside the image (so every output is
+    // 4 px or more from the border: b1 = 1, e = 0); uniform over the block
+    const bool interior = x0 >= 8 && x0 + kMxWaves * kMxCols + 8 <= G.w && G.y0 >= 4 && G.y0 + kMxRows + 4 <= G.h;
+    float acc[3][6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc[c][j] = 0.0f;
+    // three channels, the next one's loads in flight while one is filtered
+    auto channels = [&](auto edge_tag) {
+      constexpr bool kEdge = decltype(edge_tag)::value;
+      MxIn in0, in1;
+      mx_load<kEdge>(in0, fast_plane<_Float16>(P, s, 0, 0), fast_plane<_Float16>(P, s, 1, 0), G);
+      mx_load<kEdge>(in1, fast_plane<_Float16>(P, s, 0, 1), fast_plane<_Float16>(P, s, 1, 1), G);
+      mx_channel<kEdge>(in0, K, xyb_shift(0), G, acc[0]);
+      mx_load<kEdge>(in0, fast_plane<_Float16>(P, s, 0, 2), fast_plane<_Float16>(P, s, 1, 2), G);
+      mx_channel<kEdge>(in1, K, xyb_shift(1), G, acc[1]);
+      mx_channel<kEdge>(in0, K, xyb_shift(2), G, acc[2]);
+    };
+    if (interior)
+      channels(std::false_type{});
+    else if (G.xw < G.w)  // uniform per wave: a wave past the right edge adds zeros
+      channels(std::true_type{});
+    // Block sums in a fixed order: f32 within each wave, then f64 over the
+    // four waves in sequence.
+    float v18[18];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) v18[c * 6 + j] = acc[c][j];
+    int which;
+    const float vs = wave_sums18(v18, lane, &which);
+    fast_barrier();  // the previous item's reads of red are done
+    if (which >= 0) red[wv][which] = (double)vs;
+    fast_barrier();
+    if (tid < 18) {
+      double t = 0.0;
+      for (int q = 0; q < kMxWaves; ++q) t += red[q][tid];
+      const int c = tid / 6, j = tid % 6;
+      ((__attribute__((address_space(1))) double*)P.partials)[(long long)(S.part + c * S.colblocks + tile) * kNumSums + j] = t;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 
 // One block per tile: blocks resident together start at different times, so
@@ -728,8 +1065,14 @@ static hipError_t launch_tile_t(int tile_cfg, const PairDesc* pairs, const int* 
     case 2:
       hipLaunchKernelGGL((k_fast_tile<T, 32, 32, 2>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
-    default:
+    case 3:
       hipLaunchKernelGGL((k_fast_tile<T, 32, 24, 3>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      break;
+    default:  // 64x32: the matrix-core kernel for binary16 planes
+      if constexpr (std::is_same<T, _Float16>::value)
+        hipLaunchKernelGGL(k_fast_tile_mfma, dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
+      else
+        hipLaunchKernelGGL((k_fast_tile<T, 64, 32, 2>), dim3(grid), dim3(kThreads), 0, stream, pairs, prefix, meta);
       break;
   }
   return hipGetLastError();
@@ -741,7 +1084,12 @@ static const void* tile_kernel_t(int tile_cfg) {
     case 0: return reinterpret_cast<const void*>(&k_fast_tile<T, 32, 16, 2>);
     case 1: return reinterpret_cast<const void*>(&k_fast_tile<T, 64, 16, 2>);
     case 2: return reinterpret_cast<const void*>(&k_fast_tile<T, 32, 32, 2>);
-    default: return reinterpret_cast<const void*>(&k_fast_tile<T, 32, 24, 3>);
+    case 3: return reinterpret_cast<const void*>(&k_fast_tile<T, 32, 24, 3>);
+    default:
+      if constexpr (std::is_same<T, _Float16>::value)
+        return reinterpret_cast<const void*>(&k_fast_tile_mfma);
+      else
+        return reinterpret_cast<const void*>(&k_fast_tile<T, 64, 32, 2>);
   }
 }
 

@@ -32,9 +32,9 @@ enum Storage : int {
 };
 
 // Output tile shapes of the fast form (k_fast_tile instantiations).
-constexpr int kNumTileCfgs = 4;
-constexpr int kTileCfgW[kNumTileCfgs] = {32, 64, 32, 32};
-constexpr int kTileCfgH[kNumTileCfgs] = {16, 16, 32, 24};
+constexpr int kNumTileCfgs = 5;
+constexpr int kTileCfgW[kNumTileCfgs] = {32, 64, 32, 32, 64};
+constexpr int kTileCfgH[kNumTileCfgs] = {16, 16, 32, 24, 32};
 
 enum InputKind : int {
   kInputSrgbU8 = 0,     // packed RGB u8, pitched

@@ -37,7 +37,7 @@ __all__ = [
 ]
 
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
-TILE_SHAPES = [(32, 16), (64, 16), (32, 32), (32, 24)]
+TILE_SHAPES = [(32, 16), (64, 16), (32, 32), (32, 24), (64, 32)]
 
 STATUS = {
     0: "SSIM2_OK",
