@@ -761,28 +761,46 @@ struct MxIn {
 };
 
 // Tile geometry of a wave: its first output column xw, the tile's first row
-// y0, the scale's size and pitch.
+// y0, the scale's size and pitch; near the border also the per-item edge
+// data: the dword masks that zero a horizontal window's columns past the
+// image, and the 1-D blurs of ones of the lane's map columns and rows.
 struct MxGeo {
   int xw, y0, w, h, g, l;
   long long pitch;
+  unsigned colmask[4];
+  float b1x[4], b1y[2];
 };
+
+// Per-item edge data of a lane (see MxGeo).
+__device__ __forceinline__ void mx_edge_setup(MxGeo& G) {
+  const int xc = G.xw - 8 + 8 * G.g;
+  const int nv = min(max(G.w - xc, 0), 8);  // columns of the window inside the image
+#pragma unroll
+  for (int k = 0; k < 4; ++k) G.colmask[k] = 2 * k + 1 < nv ? 0xFFFFFFFFu : (2 * k < nv ? 0x0000FFFFu : 0u);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) G.b1x[i] = tap_sum(G.xw + 4 * G.g + i, G.w);
+#pragma unroll
+  for (int ob = 0; ob < 2; ++ob) G.b1y[ob] = tap_sum(G.y0 + 16 * ob + G.l, G.h);
+}
 
 // 8 binary16 values of row yr, columns xc .. xc + 7 (xc a multiple of 8),
 // zero outside the image. kEdge = false: the whole window lies inside the
 // image (the caller checked).
 template <bool kEdge>
 __device__ __forceinline__ mx_h8 mx_load8(const gT<_Float16>* plane, const MxGeo& G, int yr, int xc, bool used) {
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
   mx_h8 v = {};
   if (!kEdge) {
     if (used) v = *(const gT<mx_h8>*)(plane + (long long)yr * G.pitch + xc);
     return v;
   }
-  // a window starting inside the row ends inside its pitch (a multiple of 16)
+  // a window starting inside the row ends inside its pitch (a multiple of
+  // 16); its columns past the image are masked off
   if (used && yr >= 0 && yr < G.h && xc >= 0 && xc < G.w) {
-    v = *(const gT<mx_h8>*)(plane + (long long)yr * G.pitch + xc);
+    u4 raw = *(const gT<u4>*)(plane + (long long)yr * G.pitch + xc);
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (xc + j >= G.w) v[j] = (_Float16)0.0f;
+    for (int k = 0; k < 4; ++k) raw[k] &= G.colmask[k];
+    v = __builtin_bit_cast(mx_h8, raw);
   }
   return v;
 }
@@ -873,53 +891,79 @@ __device__ __forceinline__ mx_f4 mx_vpass(mx_f4 upper, mx_f4 lower, const MxTaps
   return mx_mfma(thi, K.v_hi, mx_mfma(thi, K.v_lo, mx_mfma(tlo, K.v_hi, z4)));
 }
 
+// k x - m for a binary16 x (half `hi` of the pair), in one mixed-precision fma.
+template <int hi>
+__device__ __forceinline__ float mx_scaled_diff(unsigned pair, float k, float m) {
+  float r;
+  if (hi)
+    asm("v_fma_mix_f32 %0, %1, %2, -%3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(pair), "v"(k), "v"(m));
+  else
+    asm("v_fma_mix_f32 %0, %1, %2, -%3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(pair), "v"(k), "v"(m));
+  return r;
+}
+
 // Error maps of output block ob (row y0 + 16 ob + l, columns xw + 4 g + i)
-// from the blurred quantities V and the pixels; the same formulas as
-// k_fast_tile's (zero-padding terms near the border).
+// from the blurred quantities V (means x64, Q and r'd' x4096) and the
+// pixels, the formulas of k_fast_tile rearranged so that equal operands give
+// exact zeros without comparisons:
+//   var1 + var2 - 2 cov = Q - (mu1 - mu2)^2, so den = num_s + (Q - md^2) and
+//   1 - ssim = (den - num_m num_s) / den;
+//   (1 + |dis - mu2|) / (1 + |src - mu1|) - 1 = (|e2| - |e1|) / (64 + |e1|)
+//   with e = 64 (x - mu);
+// near the border cov takes its zero-padding term (var1 + var2 - 2 cov has
+// none) and e its shift term. The six sums are kept as
+// {sum d, sum d^4, sum |d1|, sum d1, sum d1^4, sum d1^3 |d1|}: artifact and
+// detail loss are their half sums and differences (mx_finish_sums).
 template <bool kEdge>
 __device__ __forceinline__ void mx_maps(const mx_f4 (&V)[4], mx_h4 src, mx_h4 dis, float sh, const MxGeo& G, int ob,
                                         float (&a6)[6]) {
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  const u2 sp = __builtin_bit_cast(u2, src), dp = __builtin_bit_cast(u2, dis);
+  const float k64 = kMxMeanScale;
   const int gy = G.y0 + 16 * ob + G.l, gx0 = G.xw + 4 * G.g;
-  const float b1y = kEdge ? tap_sum(gy, G.h) : 1.0f;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int gx = gx0 + i;
-    // all statistics in the common scale: means x64, var and cov x4096
-    const float m1 = V[0][i], m2 = V[1][i], s12 = V[3][i], ssum = fmaf(2.0f, s12, V[2][i]);
-    float vsum = ssum - (m1 * m1 + m2 * m2), cov = s12 - m1 * m2;
+    const float m1 = V[0][i], m2 = V[1][i], q = V[2][i], s12 = V[3][i];
+    float cov = s12 - m1 * m2;
     float ec = 0.0f;
     if constexpr (kEdge) {
-      if (gx >= G.w || gy >= G.h) continue;
-      const float b1 = tap_sum(gx, G.w) * b1y;
+      if (gx0 + i >= G.w || gy >= G.h) continue;
+      const float b1 = G.b1x[i] * G.b1y[ob];
       ec = (1.0f - b1) * (kMxMeanScale * sh);
-      const float t = (m1 + m2) + (kMxMeanScale * sh) * b1;
-      vsum = fmaf(2.0f * ec, t, vsum);
-      cov = fmaf(ec, t, cov);
+      cov = fmaf(ec, (m1 + m2) + (kMxMeanScale * sh) * b1, cov);
     }
-    const float md = m1 - m2;
-    const float num_m = fmaf(md, md * (-1.0f / kMxVarScale), 1.0f);
+    const float md = m1 - m2, md2 = md * md;
+    const float num_m = fmaf(md2, -1.0f / kMxVarScale, 1.0f);
     const float num_s = fmaf(2.0f, cov, kMxVarScale * kC2);
-    const float den = vsum + kMxVarScale * kC2;
-    const float nm = num_m * num_s;
-    const float dd = fmaxf(1.0f - (nm == den ? 1.0f : nm * __builtin_amdgcn_rcpf(den)), 0.0f);
+    const float den = num_s + (q - md2);
+    const float dd = fmaxf((den - num_m * num_s) * __builtin_amdgcn_rcpf(den), 0.0f);
     const float d2 = dd * dd;
     a6[0] += dd;
     a6[1] = fmaf(d2, d2, a6[1]);
-    // x - mu scaled by 64, and 1 + |x - mu| = fma(|64 (x - mu)|, 1/64, 1)
-    float rn = fmaf((float)dis[i], kMxMeanScale, -m2), rd = fmaf((float)src[i], kMxMeanScale, -m1);
+    const unsigned sw = i < 2 ? sp[0] : sp[1], dw = i < 2 ? dp[0] : dp[1];
+    float e2 = (i & 1) ? mx_scaled_diff<1>(dw, k64, m2) : mx_scaled_diff<0>(dw, k64, m2);
+    float e1 = (i & 1) ? mx_scaled_diff<1>(sw, k64, m1) : mx_scaled_diff<0>(sw, k64, m1);
     if constexpr (kEdge) {
-      rn += ec;
-      rd += ec;
+      e2 += ec;
+      e1 += ec;
     }
-    const float en = fmaf(fabsf(rn), 1.0f / kMxMeanScale, 1.0f), ed = fmaf(fabsf(rd), 1.0f / kMxMeanScale, 1.0f);
-    const float d1 = en == ed ? 0.0f : fmaf(en, __builtin_amdgcn_rcpf(ed), -1.0f);
-    const float art = fmaxf(d1, 0.0f), det = fmaxf(-d1, 0.0f);
-    const float a2 = art * art, e2 = det * det;
-    a6[2] += art;
-    a6[3] = fmaf(a2, a2, a6[3]);
-    a6[4] += det;
-    a6[5] = fmaf(e2, e2, a6[5]);
+    const float d1 = (fabsf(e2) - fabsf(e1)) * __builtin_amdgcn_rcpf(k64 + fabsf(e1));
+    const float d1s = d1 * d1;
+    a6[2] += fabsf(d1);
+    a6[3] += d1;
+    a6[4] = fmaf(d1s, d1s, a6[4]);
+    a6[5] = fmaf(d1s, d1 * fabsf(d1), a6[5]);
   }
+}
+
+// The kept sums of mx_maps in the order of the other tile kernels: ssim
+// d and d^4, artifact and its fourth powers, detail loss and its.
+__device__ __forceinline__ void mx_finish_sums(float (&a6)[6]) {
+  const float abs1 = a6[2], sgn1 = a6[3], p4 = a6[4], sgn4 = a6[5];
+  a6[2] = 0.5f * (abs1 + sgn1);
+  a6[3] = 0.5f * (p4 + sgn4);
+  a6[4] = 0.5f * (abs1 - sgn1);
+  a6[5] = 0.5f * (p4 - sgn4);
 }
 
 // One channel of a wave: row blocks 0 and 1, output block 0, then row block
@@ -998,10 +1042,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
       mx_channel<kEdge>(in1, K, xyb_shift(1), G, acc[1]);
       mx_channel<kEdge>(in0, K, xyb_shift(2), G, acc[2]);
     };
-    if (interior)
+    if (interior) {
       channels(std::false_type{});
-    else if (G.xw < G.w)  // uniform per wave: a wave past the right edge adds zeros
+    } else if (G.xw < G.w) {  // uniform per wave: a wave past the right edge adds zeros
+      mx_edge_setup(G);
       channels(std::true_type{});
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mx_finish_sums(acc[c]);
     // Block sums in a fixed order: f32 within each wave, then f64 over the
     // four waves in sequence.
     float v18[18];
