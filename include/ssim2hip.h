@@ -285,8 +285,9 @@ ssim2_status ssim2_batch_query(ssim2_batch* batch, int* done);
 ssim2_status ssim2_batch_get_sums(ssim2_batch* batch, uint32_t index, double* sums108, int* dims12);
 /* Output tile shape (0: 32x16, 1: 64x16, 2: 32x32, 3: 32x24, 4: 64x32) and
  * grid size in multiples of the CU count (0: one block per tile, sized per
- * batch); ssim2_batch_autotune times every combination on the given pairs and
- * keeps the fastest. With binary16 storage, shape 4 runs the blur on the
+ * batch); a new engine starts at shape 4, grid 32 with binary16 storage and
+ * shape 0, grid 64 with f32. ssim2_batch_autotune times every combination on
+ * the given pairs and keeps the fastest. With binary16 storage, shape 4 runs the blur on the
  * matrix cores (binary16 operands split into hi + lo parts, f32 sums; scores
  * within about 1e-3 of the other shapes) and assumes linear inputs in [0, 1]
  * as sRGB and YUV inputs always are: its blurred squares stay finite for
@@ -313,7 +314,8 @@ typedef struct {
   uint32_t max_pairs_per_batch;
   uint64_t max_pixels_per_batch;  /* also the largest request accepted */
   uint64_t hbm_budget;            /* admission limit on in-flight device bytes */
-  uint32_t tile_cfg;              /* as ssim2_batch_set_tiling; 0/0 = default */
+  uint32_t tile_cfg;              /* as ssim2_batch_set_tiling; 0/0 = the batch engine's
+                                     default shape (4 for binary16, 0 for f32), grid 8 x CUs */
   uint32_t grid_mult;
 } ssim2_server_options;
 

@@ -217,6 +217,7 @@ def test_autotune_keeps_scores(s2, mix, storage):
     _, dev = mix
     _say(f"[gpu] autotune, {storage}")
     with s2.Ssim2Batch(max_pairs=64, max_pixels=4_000_000, storage=storage) as b:
+        b.set_tiling(0, 64)  # the reference configuration of the comparisons below
         before = b.compute(dev)
         rep = b.autotune(dev, reps=2)
         _say(rep)

@@ -427,6 +427,10 @@ ssim2_status ssim2_batch_create(const ssim2_batch_options* o, ssim2_batch** out)
   ssim2_batch* b = new (std::nothrow) ssim2_batch();
   if (!b) return bfail(SSIM2_OOM, "host allocation failed");
   b->storage = (int)o->storage;
+  if (b->storage == kStorageF16) {  // the matrix-core tiles, a few blocks per CU's worth of items each
+    b->tile_cfg = 4;
+    b->grid_mult = 32;
+  }
   b->max_pairs = o->max_pairs;
   b->max_pixels = o->max_pixels;
   const size_t sb = storage_bytes(b->storage);

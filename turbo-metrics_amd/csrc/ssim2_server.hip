@@ -283,8 +283,11 @@ ssim2_status ssim2_server_create(const ssim2_server_options* o, ssim2_server** o
       delete sv;
       return sfail(st != SSIM2_OK ? st : SSIM2_HIP_ERROR, "server slot: " + m);
     }
-    // serving batches are small: a smaller grid unless the caller sets one
-    ssim2_batch_set_tiling(s.batch, o->tile_cfg, o->grid_mult ? o->grid_mult : 8);
+    // serving batches are small: a smaller grid unless the caller sets one;
+    // tile shape 0 with grid 0 keeps the engine's default shape
+    uint32_t cfg = o->tile_cfg, mult = 0;
+    if (cfg == 0 && o->grid_mult == 0) ssim2_batch_get_tiling(s.batch, &cfg, &mult);
+    ssim2_batch_set_tiling(s.batch, cfg, o->grid_mult ? o->grid_mult : 8);
   }
   if (hipStreamCreateWithFlags(&sv->reg_stream, hipStreamNonBlocking) != hipSuccess) {
     for (Slot& t : sv->slots) {
