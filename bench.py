@@ -197,6 +197,7 @@ def run_batch(s2, workloads, storage, host_pairs, spec, reps):
         "value": round(px / 1e6 / dt, 1),
         "unit": "Mpx/s",
         "tile": list(s2.TILE_SHAPES[cfg]),
+        "tile_kernel": "matrix cores (binary16 hi/lo operands)" if (cfg == 4 and storage == "f16") else "VALU FIR",
         "grid_x_cu": mult,
         "resident_ref": {"ms_per_batch": round(dt_ref * 1e3, 4), "value": round(px / 1e6 / dt_ref, 1),
                          "unit": "Mpx/s", "note": "reference pyramids resident, same scores"},
