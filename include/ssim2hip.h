@@ -289,9 +289,9 @@ ssim2_status ssim2_batch_get_sums(ssim2_batch* batch, uint32_t index, double* su
  * shape 0, grid 64 with f32. ssim2_batch_autotune times every combination on
  * the given pairs and keeps the fastest. With binary16 storage, shape 4 runs the blur on the
  * matrix cores (binary16 operands split into hi + lo parts, f32 sums; scores
- * within about 1e-3 of the other shapes) and assumes linear inputs in [0, 1]
- * as sRGB and YUV inputs always are: its blurred squares stay finite for
- * shifted XYB values up to about 11. `report` (optional) receives a table. */
+ * within about 1e-3 of the other shapes); its scaled squares stay finite for
+ * shifted XYB values up to about 11, and the sRGB and YUV inputs of the batch
+ * give at most 0.45. `report` (optional) receives a table. */
 ssim2_status ssim2_batch_set_tiling(ssim2_batch* batch, uint32_t tile_cfg, uint32_t grid_mult);
 ssim2_status ssim2_batch_get_tiling(const ssim2_batch* batch, uint32_t* tile_cfg, uint32_t* grid_mult);
 ssim2_status ssim2_batch_autotune(ssim2_batch* batch, const ssim2_pair* pairs, uint32_t npairs,
