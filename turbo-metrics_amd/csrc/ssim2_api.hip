@@ -46,8 +46,11 @@ struct ssim2_ctx {
   PairPlan plan;
   float* arena = nullptr;
   double* partials = nullptr;
-  double* result = nullptr;       // device, kResultDoubles
-  double* result_host = nullptr;  // pinned
+  // The result record (kResultDoubles) in pinned, device-mapped host memory:
+  // the finalisation kernel writes it there directly, so a pair needs no
+  // copy command after its kernels; `result` is its device address.
+  double* result = nullptr;
+  double* result_host = nullptr;
   PairDesc* d_desc = nullptr;
   WorkItem* d_items = nullptr;
   unsigned char* stage = nullptr;  // two packed u8 images for host uploads
@@ -68,7 +71,6 @@ namespace {
 void release(ssim2_ctx* c) {
   if (c->arena) (void)hipFree(c->arena);
   if (c->partials) (void)hipFree(c->partials);
-  if (c->result) (void)hipFree(c->result);
   if (c->result_host) (void)hipHostFree(c->result_host);
   if (c->d_desc) (void)hipFree(c->d_desc);
   if (c->d_items) (void)hipFree(c->d_items);
@@ -166,9 +168,7 @@ ssim2_status enqueue(ssim2_ctx* c, int sides, const InputArgs& in, hipStream_t s
   HIP_TRY(launch_rows(c->d_desc, c->d_items + p.off_rows[sides], p.n_rows[sides], sides, s));
   HIP_TRY(launch_cols(c->d_desc, c->d_items + p.off_cols, p.n_cols, sides, s));
   if (sides != kSidesRef) {
-    HIP_TRY(launch_finalize(c->d_desc, 1, s));
-    HIP_TRY(hipMemcpyAsync(c->result_host, c->result, kResultDoubles * sizeof(double),
-                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(launch_finalize(c->d_desc, 1, s));  // writes the host record (result_host)
     c->have_result = true;
   }
   return mark_issued(c, s);
@@ -242,9 +242,9 @@ ssim2_status ssim2_create(uint32_t width, uint32_t height, const ssim2_options* 
   hipError_t e = hipSuccess;
   if ((e = hipMalloc(&c->arena, arena_bytes)) != hipSuccess ||
       (e = hipMalloc(&c->partials, std::max<size_t>(part_bytes, 8))) != hipSuccess ||
-      (e = hipMalloc(&c->result, kResultDoubles * sizeof(double))) != hipSuccess ||
-      (e = hipHostMalloc(&c->result_host, kResultDoubles * sizeof(double), hipHostMallocDefault)) !=
-          hipSuccess ||
+      (e = hipHostMalloc(&c->result_host, kResultDoubles * sizeof(double),
+                         hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess ||
+      (e = hipHostGetDevicePointer((void**)&c->result, c->result_host, 0)) != hipSuccess ||
       (e = hipMalloc(&c->d_desc, sizeof(PairDesc))) != hipSuccess ||
       (e = hipMalloc(&c->d_items, items_bytes)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&c->done, hipEventDisableTiming)) != hipSuccess) {
