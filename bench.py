@@ -49,6 +49,13 @@ HBM_PEAK_GBS = 8000.0
 W, H, SEED, STEP = 1920, 1080, 2, 16.0
 
 
+# Largest batch of the serving and mixed-stream legs: 1024 pairs, 128 Mpx
+# (about 2.2 GB of binary16 planes per engine). Measured on the mixed-size
+# stream with two engines: 64 Mpx batches 43.0, 128 Mpx 45.1, 256 Mpx 45.3
+# Gpx/s; larger launches have fewer ramp and tail phases.
+BATCH_PAIRS, BATCH_PIXELS = 1024, 128 << 20
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -225,8 +232,8 @@ def run_serving(s2, workloads, seconds, rate, storage="f16", slots=3, resident_r
     dev = [(s2.DeviceImage.from_host(r), s2.DeviceImage.from_host(d)) for r, d in host]
     px_of = [w * h for _, _, w, h, _ in spec]
     t_arr, pick = workloads.serving_schedule(rate, seconds, pool_per_class=per_class)
-    sv = s2.Ssim2Server(storage=storage, slots=slots, max_pairs_per_batch=512,
-                        max_pixels_per_batch=64 << 20, hbm_budget=64 << 30)
+    sv = s2.Ssim2Server(storage=storage, slots=slots, max_pairs_per_batch=BATCH_PAIRS,
+                        max_pixels_per_batch=BATCH_PIXELS, hbm_budget=64 << 30)
     # warm-up: every pool pair once (graph capture, caches)
     for i, p in enumerate(dev):
         sv.submit(*p, 10_000_000 + i)
@@ -287,7 +294,7 @@ def run_mixed(s2, workloads, pool_dev, spec, cls, seconds=1.0, rate=2000.0, reps
     """Throughput on the mixed-size request stream (the first `seconds` of the
     Poisson schedule, all requests present at once): the plain exact f32 path
     pair by pair (one context per size, async on one stream) against the
-    batched fast form with up to 512 pairs / 64 Mpx per launch (two batch
+    batched fast form with up to 1024 pairs / 128 Mpx per launch (two batch
     engines alternating, each batch's scores read after the next is
     launched)."""
     per_class = [cls.count(c) for c in range(4)]
@@ -312,11 +319,11 @@ def run_mixed(s2, workloads, pool_dev, spec, cls, seconds=1.0, rate=2000.0, reps
     dt_plain = (time.perf_counter() - t0) / reps
     for c in ctxs.values():
         c.close()
-    # batches as the server forms them: FIFO, at most 512 pairs and 64 Mpx
+    # batches as the server forms them: FIFO, at most BATCH_PAIRS pairs and BATCH_PIXELS
     batches, cur, cur_px = [], [], 0
     for k in reqs:
         p = spec[k][2] * spec[k][3]
-        if cur and (len(cur) == 512 or cur_px + p > (64 << 20)):
+        if cur and (len(cur) == BATCH_PAIRS or cur_px + p > BATCH_PIXELS):
             batches.append(cur)
             cur, cur_px = [], 0
         cur.append(k)
@@ -325,7 +332,7 @@ def run_mixed(s2, workloads, pool_dev, spec, cls, seconds=1.0, rate=2000.0, reps
         batches.append(cur)
     # two engines on two streams, alternating: batch k+1 is staged and
     # launched before batch k's scores are read, as a throughput client would
-    engines = [s2.Ssim2Batch(max_pairs=512, max_pixels=64 << 20, storage=storage) for _ in range(2)]
+    engines = [s2.Ssim2Batch(max_pairs=BATCH_PAIRS, max_pixels=BATCH_PIXELS, storage=storage) for _ in range(2)]
     streams = [stream, s2.Stream()]
     engines[0].autotune([pool_dev[k] for k in batches[0]], reps=2, stream=stream)
     engines[1].set_tiling(*engines[0].tiling())

@@ -244,7 +244,7 @@ struct TileBytes {
 // ((x0,y0) + (x1,y0)) + (x0,y1)) + (x1,y1) of cpu.rs:545-579; only scales
 // >= 1 are kept in LDS.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __restrict__ pairs,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_fast_pyramid(const PairDesc* __restrict__ pairs,
                                                            const int* __restrict__ prefix,
                                                            const int* __restrict__ meta) {
   constexpr int T0 = kPyrTile, T1 = T0 / 2;
@@ -260,44 +260,66 @@ __global__ __launch_bounds__(kThreads) void k_fast_pyramid(const PairDesc* __res
   // Scales 2.. (at most 8x8 pixels) of an item run on one wave after the
   // next item's barrier, rotating over the items: a wave's LDS accesses take
   // effect in order, so the levels need no block barrier of their own, and
-  // the other waves meanwhile work on the next item.
+  // the other waves meanwhile work on the next item. Scale 2 takes a lane per
+  // pixel with both images side by side; scales 3.. (at most 16 pixels per
+  // image) a lane per (image, pixel).
+  auto level_pixel = [&](const PairDesc& Q, int s, int img, int gx, int gy, const float (&rgb)[3]) {
+    float X, Y, B;
+    xyb_fast_shifted(rgb[0], rgb[1], rgb[2], X, Y, B);
+    const long long off = (long long)gy * Q.sc[s].pitch + gx;
+    fast_plane<T>(Q, s, img, 0)[off] = Codec<T>::enc(X);
+    fast_plane<T>(Q, s, img, 1)[off] = Codec<T>::enc(Y);
+    fast_plane<T>(Q, s, img, 2)[off] = Codec<T>::enc(B);
+  };
   auto levels = [&](const PairDesc& Q, int tx, int ty, float (*B1)[T1][T1 + 1]) {
-    for (int s = 2; s < Q.nscales; ++s) {
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      const int Ts = T0 >> s;
-      const int pw = Q.sc[s - 1].w, ph = Q.sc[s - 1].h;
-      const int cw = Q.sc[s].w, ch = Q.sc[s].h;
-      const int ox0 = tx * Ts, oy0 = ty * Ts;
+    {
+      // scale 2 from the scale-1 buffer
+      constexpr int Ts = T0 >> 2;
+      const int pw = Q.sc[1].w, ph = Q.sc[1].h;
       const int lx = lane % Ts, ly = lane / Ts;
+      const int gx = tx * Ts + lx, gy = ty * Ts + ly;
+      if (gx < Q.sc[2].w && gy < Q.sc[2].h) {
+        const int x0 = min(2 * gx, pw - 1) - 2 * tx * Ts, x1 = min(2 * gx + 1, pw - 1) - 2 * tx * Ts;
+        const int y0 = min(2 * gy, ph - 1) - 2 * ty * Ts, y1 = min(2 * gy + 1, ph - 1) - 2 * ty * Ts;
+        float rgb[2][3];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const float v = (((B1[k][y0][x0] + B1[k][y0][x1]) + B1[k][y1][x0]) + B1[k][y1][x1]) * 0.25f;
+          rgb[k / 3][k % 3] = v;
+          bufA[k][ly][lx] = v;
+        }
+#pragma unroll
+        for (int img = 0; img < 2; ++img)
+          if (!((Q.skip_img >> img) & 1)) level_pixel(Q, 2, img, gx, gy, rgb[img]);
+      }
+    }
+    for (int s = 3; s < Q.nscales; ++s) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const int Ts = T0 >> s, np = Ts * Ts;
+      const int pw = Q.sc[s - 1].w, ph = Q.sc[s - 1].h;
+      const int ox0 = tx * Ts, oy0 = ty * Ts;
+      const int img = lane / np, li = lane % np;
+      const int lx = li % Ts, ly = li / Ts;
       const int gx = ox0 + lx, gy = oy0 + ly;
-      if (lane >= Ts * Ts || gx >= cw || gy >= ch) continue;
+      if (img >= 2 || ((Q.skip_img >> img) & 1) || gx >= Q.sc[s].w || gy >= Q.sc[s].h) continue;
       const int x0 = min(2 * gx, pw - 1) - 2 * ox0, x1 = min(2 * gx + 1, pw - 1) - 2 * ox0;
       const int y0 = min(2 * gy, ph - 1) - 2 * oy0, y1 = min(2 * gy + 1, ph - 1) - 2 * oy0;
+      float rgb[3];
 #pragma unroll
-      for (int img = 0; img < 2; ++img) {
-        if ((Q.skip_img >> img) & 1) continue;
-        float rgb[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const int k = img * 3 + c;
-          float sum;
-          if (s & 1)
-            sum = ((bufA[k][y0][x0] + bufA[k][y0][x1]) + bufA[k][y1][x0]) + bufA[k][y1][x1];
-          else
-            sum = ((B1[k][y0][x0] + B1[k][y0][x1]) + B1[k][y1][x0]) + B1[k][y1][x1];
-          rgb[c] = sum * 0.25f;
-          if (s & 1)
-            B1[k][ly][lx] = rgb[c];
-          else
-            bufA[k][ly][lx] = rgb[c];
-        }
-        float X, Y, B;
-        xyb_fast_shifted(rgb[0], rgb[1], rgb[2], X, Y, B);
-        const long long off = (long long)gy * Q.sc[s].pitch + gx;
-        fast_plane<T>(Q, s, img, 0)[off] = Codec<T>::enc(X);
-        fast_plane<T>(Q, s, img, 1)[off] = Codec<T>::enc(Y);
-        fast_plane<T>(Q, s, img, 2)[off] = Codec<T>::enc(B);
+      for (int c = 0; c < 3; ++c) {
+        const int k = img * 3 + c;
+        float sum;
+        if (s & 1)
+          sum = ((bufA[k][y0][x0] + bufA[k][y0][x1]) + bufA[k][y1][x0]) + bufA[k][y1][x1];
+        else
+          sum = ((B1[k][y0][x0] + B1[k][y0][x1]) + B1[k][y1][x0]) + B1[k][y1][x1];
+        rgb[c] = sum * 0.25f;
+        if (s & 1)
+          B1[k][ly][lx] = rgb[c];
+        else
+          bufA[k][ly][lx] = rgb[c];
       }
+      level_pixel(Q, s, img, gx, gy, rgb);
     }
   };
   int pend = -1, pend_tx = 0, pend_ty = 0, pend_nb = 0;  // item whose levels are due
