@@ -737,8 +737,9 @@ void k_fast_tile(const PairDesc* __restrict__ pairs,
 // binary16 planes as hi + lo exactly (hi = f16(x y); lo = fma(x, y, -hi) in
 // binary16 is the exact remainder), so blur(r'^2 + d'^2) is the sum of
 // hi.Whi + hi.Wlo + lo.Whi over both squares; the horizontal results T as
-// hi + lo likewise before the vertical pass. The lo.lo products (below
-// 2^-22 relative) are dropped.
+// hi + lo before the vertical pass (hi rounded to nearest, the exact
+// remainder rounded toward zero: T - hi - lo below 2^-21 relative). The
+// lo.lo products (below 2^-21 relative) are dropped.
 typedef _Float16 mx_h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 mx_h4 __attribute__((ext_vector_type(4)));
 typedef float mx_f4 __attribute__((ext_vector_type(4)));
@@ -772,6 +773,9 @@ __device__ __forceinline__ void mx_tap(int t, float scale, _Float16& hi, _Float1
 // and vertical band Wv (lane (g, r = l): rho = 4 g + j, or 16 + 4 g + j - 4).
 struct MxTaps {
   mx_h8 h_hi, h_lo, m_hi, m_lo, v_hi, v_lo;  // horizontal (squares), horizontal (means), vertical
+  // -1 and 0 as values the compiler cannot see through (mx_split)
+  float n1;
+  unsigned z;
 };
 
 // One channel's inputs of a wave: the horizontal A operands of row blocks
@@ -857,27 +861,34 @@ __device__ __forceinline__ unsigned mx_pk(float x, float y) {
 }
 
 // The remainders x - hi.lo and y - hi.hi in f32 (exact: hi is x, y rounded
-// to binary16), one mixed-precision fma each with the binary16 operand read
-// in place; packed to binary16 again.
-__device__ __forceinline__ unsigned mx_rem(unsigned hi, float x, float y) {
-  float rx, ry;
-  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(rx) : "v"(hi), "v"(x));
-  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(ry) : "v"(hi), "v"(y));
-  return mx_pk(rx, ry);
+// to binary16), one mixed-precision fma each (v_fma_mix_f32, the binary16
+// operand read in place; n1 = -1), packed to binary16 rounding toward zero.
+__device__ __forceinline__ unsigned mx_rem(unsigned hi, float x, float y, float n1) {
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  const h2 h = __builtin_bit_cast(h2, hi);
+  return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(fmaf((float)h.x, n1, x), fmaf((float)h.y, n1, y)));
 }
 
 // hi and lo binary16 parts of the f32 values u[0..3], v[0..3] as one operand.
-__device__ __forceinline__ void mx_split(mx_f4 u, mx_f4 v, mx_h8& hi, mx_h8& lo) {
+// Plain code, no inline assembly: the compiler must see every instruction
+// that reads or writes a register near the matrix products, since it alone
+// inserts the wait states they require (a matrix product reads its
+// accumulator over several passes; inline assembly is not checked against
+// that). The opaque z (0) keeps each packed hi word whole, so the remainders
+// read its halves in place instead of converting x and y a second time; the
+// opaque n1 (-1) keeps each remainder an fma rather than a subtraction,
+// which has no mixed-precision form.
+__device__ __forceinline__ void mx_split(mx_f4 u, mx_f4 v, float n1, unsigned z, mx_h8& hi, mx_h8& lo) {
   typedef unsigned u4 __attribute__((ext_vector_type(4)));
   u4 h, l;
-  h[0] = mx_pk(u[0], u[1]);
-  h[1] = mx_pk(u[2], u[3]);
-  h[2] = mx_pk(v[0], v[1]);
-  h[3] = mx_pk(v[2], v[3]);
-  l[0] = mx_rem(h[0], u[0], u[1]);
-  l[1] = mx_rem(h[1], u[2], u[3]);
-  l[2] = mx_rem(h[2], v[0], v[1]);
-  l[3] = mx_rem(h[3], v[2], v[3]);
+  h[0] = mx_pk(u[0], u[1]) ^ z;
+  h[1] = mx_pk(u[2], u[3]) ^ z;
+  h[2] = mx_pk(v[0], v[1]) ^ z;
+  h[3] = mx_pk(v[2], v[3]) ^ z;
+  l[0] = mx_rem(h[0], u[0], u[1], n1);
+  l[1] = mx_rem(h[1], u[2], u[3], n1);
+  l[2] = mx_rem(h[2], v[0], v[1], n1);
+  l[3] = mx_rem(h[3], v[2], v[3], n1);
   hi = __builtin_bit_cast(mx_h8, h);
   lo = __builtin_bit_cast(mx_h8, l);
 }
@@ -909,19 +920,16 @@ __device__ __forceinline__ void mx_hpass(const mx_h8& a, const mx_h8& d, const M
 __device__ __forceinline__ mx_f4 mx_vpass(mx_f4 upper, mx_f4 lower, const MxTaps& K) {
   const mx_f4 z4 = {0.0f, 0.0f, 0.0f, 0.0f};
   mx_h8 thi, tlo;
-  mx_split(upper, lower, thi, tlo);
+  mx_split(upper, lower, K.n1, K.z, thi, tlo);
   return mx_mfma(thi, K.v_hi, mx_mfma(thi, K.v_lo, mx_mfma(tlo, K.v_hi, z4)));
 }
 
 // k x - m for a binary16 x (half `hi` of the pair), in one mixed-precision fma.
 template <int hi>
 __device__ __forceinline__ float mx_scaled_diff(unsigned pair, float k, float m) {
-  float r;
-  if (hi)
-    asm("v_fma_mix_f32 %0, %1, %2, -%3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(pair), "v"(k), "v"(m));
-  else
-    asm("v_fma_mix_f32 %0, %1, %2, -%3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(pair), "v"(k), "v"(m));
-  return r;
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  const h2 h = __builtin_bit_cast(h2, pair);
+  return fmaf((float)(hi ? h.y : h.x), k, -m);
 }
 
 // Error maps of output block ob (row y0 + 16 ob + l, columns xw + 4 g + i)
@@ -1017,6 +1025,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
   G.g = lane >> 4;
   G.l = lane & 15;
   MxTaps K;
+  K.n1 = -1.0f;
+  K.z = 0u;
+  asm volatile("" : "+s"(K.n1), "+s"(K.z));
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     _Float16 hi, lo;
