@@ -611,6 +611,9 @@ __global__ __launch_bounds__(ColWaves<SIDES>::threads) void k_cols(
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nch = (h + kRadius - 1 + RY - 1) / RY;  // chunks: rows -kRadius+1 .. h-1
+  // The scale-0 strips are the longest scans and set the pass's length: their
+  // waves issue first on a SIMD they share with a smaller scale's block.
+  if (s == 0) __builtin_amdgcn_s_setprio(2);
   const int y00 = -kRadius + 1;
 
   if (wv < NW) {
