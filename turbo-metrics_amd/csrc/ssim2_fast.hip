@@ -788,69 +788,19 @@ struct MxIn {
 
 // Tile geometry of a wave: its first output column xw, the tile's first row
 // y0, the scale's size and pitch; near the border also the per-item edge
-// data: the dword masks that zero a horizontal window's columns past the
-// image, and the 1-D blurs of ones of the lane's map columns and rows.
+// data: the 1-D blurs of ones of the lane's map columns and rows.
 struct MxGeo {
   int xw, y0, w, h, g, l;
   long long pitch;
-  unsigned colmask[4];
   float b1x[4], b1y[2];
 };
 
 // Per-item edge data of a lane (see MxGeo).
 __device__ __forceinline__ void mx_edge_setup(MxGeo& G) {
-  const int xc = G.xw - 8 + 8 * G.g;
-  const int nv = min(max(G.w - xc, 0), 8);  // columns of the window inside the image
-#pragma unroll
-  for (int k = 0; k < 4; ++k) G.colmask[k] = 2 * k + 1 < nv ? 0xFFFFFFFFu : (2 * k < nv ? 0x0000FFFFu : 0u);
 #pragma unroll
   for (int i = 0; i < 4; ++i) G.b1x[i] = tap_sum(G.xw + 4 * G.g + i, G.w);
 #pragma unroll
   for (int ob = 0; ob < 2; ++ob) G.b1y[ob] = tap_sum(G.y0 + 16 * ob + G.l, G.h);
-}
-
-// 8 binary16 values of row yr, columns xc .. xc + 7 (xc a multiple of 8),
-// zero outside the image. kEdge = false: the whole window lies inside the
-// image (the caller checked).
-template <bool kEdge>
-__device__ __forceinline__ mx_h8 mx_load8(const gT<_Float16>* plane, const MxGeo& G, int yr, int xc, bool used) {
-  typedef unsigned u4 __attribute__((ext_vector_type(4)));
-  mx_h8 v = {};
-  if (!kEdge) {
-    if (used) v = *(const gT<mx_h8>*)(plane + (long long)yr * G.pitch + xc);
-    return v;
-  }
-  // a window starting inside the row ends inside its pitch (a multiple of
-  // 16); its columns past the image are masked off
-  if (used && yr >= 0 && yr < G.h && xc >= 0 && xc < G.w) {
-    u4 raw = *(const gT<u4>*)(plane + (long long)yr * G.pitch + xc);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) raw[k] &= G.colmask[k];
-    v = __builtin_bit_cast(mx_h8, raw);
-  }
-  return v;
-}
-
-template <bool kEdge>
-__device__ __forceinline__ void mx_load(MxIn& in, const gT<_Float16>* pr, const gT<_Float16>* pd, const MxGeo& G) {
-#pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    const int yr = G.y0 - 4 + 16 * b + G.l, xc = G.xw - 8 + 8 * G.g;
-    const bool used = b < 2 || G.l < 8;  // rows past y0 + 35 carry weight 0
-    in.a[b] = mx_load8<kEdge>(pr, G, yr, xc, used);
-    in.d[b] = mx_load8<kEdge>(pd, G, yr, xc, used);
-  }
-#pragma unroll
-  for (int ob = 0; ob < 2; ++ob) {
-    const int gy = G.y0 + 16 * ob + G.l, gx0 = G.xw + 4 * G.g;
-    in.src[ob] = mx_h4{};
-    in.dis[ob] = mx_h4{};
-    // a 4-column group starting inside the row ends inside its pitch
-    if (!kEdge || (gy < G.h && gx0 < G.w)) {
-      in.src[ob] = *(const gT<mx_h4>*)(pr + (long long)gy * G.pitch + gx0);
-      in.dis[ob] = *(const gT<mx_h4>*)(pd + (long long)gy * G.pitch + gx0);
-    }
-  }
 }
 
 // Two f32 values rounded to binary16, packed (one v_cvt_pk_f16_f32).
@@ -1013,11 +963,107 @@ __device__ __forceinline__ void mx_channel(const MxIn& in, const MxTaps& K, floa
   mx_maps<kEdge>(V, in.src[1], in.dis[1], sh, G, 1, a6);
 }
 
+// ---- inputs through LDS by DMA
+// A wave's inputs of one channel are its six horizontal A operands (row
+// blocks 0..2, both images), each one 16-byte load per lane; the wave loads
+// them straight into its LDS buffer (global_load_lds_dwordx4: lane i's 16
+// bytes land in slot i of the operand's 1 KB). A step reads its operands
+// from the buffer into registers and then issues the next step's DMA, which
+// lands while the step is filtered: no register holds inputs in flight, and
+// the next item's first channel loads during the current item's last. (The
+// compiler's wait insertion makes any LDS read wait for every DMA in flight,
+// so the reads come before the next DMA.) The addresses are clamped into the
+// plane; what lies outside the image is cleared when an operand is read
+// (edge tiles only: an interior tile's used rows and columns are all inside,
+// and its unused rows past y0 + 35 carry weight 0). The map pixels are read
+// from the operands that contain them.
+constexpr int kMxOps = 6;
+typedef unsigned mx_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned mx_u2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) mx_u4 lds_u4;
+typedef __attribute__((address_space(3))) const mx_u4 lds_cu4;
+typedef __attribute__((address_space(3))) const mx_u2 lds_cu2;
+typedef __attribute__((address_space(1))) const void gcvoid;
+typedef __attribute__((address_space(4))) const PairDesc CPairDesc;
+
+// Issues the six DMA loads of one channel step into `dst` (kMxOps x 64 slots).
+__device__ __forceinline__ void mx_dma(lds_u4* dst, const gT<_Float16>* pr, const gT<_Float16>* pd, const MxGeo& Gs) {
+  const int xc = min(max(Gs.xw - 8 + 8 * Gs.g, 0), (int)Gs.pitch - 8);
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int yr = min(max(Gs.y0 - 4 + 16 * b + Gs.l, 0), Gs.h - 1);
+    const long long o = (long long)yr * Gs.pitch + xc;
+    __builtin_amdgcn_global_load_lds((gcvoid*)(pr + o), (void __attribute__((address_space(3)))*)(dst + (2 * b) * 64), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((gcvoid*)(pd + o), (void __attribute__((address_space(3)))*)(dst + (2 * b + 1) * 64), 16, 0, 0);
+  }
+}
+
+// Operand of row block b, image img, as the A operand of the horizontal pass.
+template <bool kEdge>
+__device__ __forceinline__ mx_h8 mx_op(lds_cu4* src, int b, int img, const MxGeo& G, int lane) {
+  mx_u4 u = src[(2 * b + img) * 64 + lane];
+  if constexpr (kEdge) {
+    const int yr = G.y0 - 4 + 16 * b + G.l, xc = G.xw - 8 + 8 * G.g;
+    const int nv = (yr >= 0 && yr < G.h && xc >= 0) ? G.w - xc : 0;  // columns inside the image
+#pragma unroll
+    for (int k = 0; k < 4; ++k) u[k] &= 2 * k + 1 < nv ? 0xFFFFFFFFu : (2 * k < nv ? 0x0000FFFFu : 0u);
+  }
+  return __builtin_bit_cast(mx_h8, u);
+}
+
+// Map pixels of output block ob (row y0 + 16 ob + l, columns xw + 4 g .. + 3)
+// of image img, from the operand slot that holds them.
+__device__ __forceinline__ mx_h4 mx_px(lds_cu4* src, int ob, int img, const MxGeo& G) {
+  const int b = G.l < 12 ? ob : ob + 1, lp = G.l < 12 ? G.l + 4 : G.l - 12;
+  lds_cu2* p = (lds_cu2*)(src + (2 * b + img) * 64 + 16 * (1 + (G.g >> 1)) + lp);
+  return __builtin_bit_cast(mx_h4, p[G.g & 1]);
+}
+
+// One channel step's inputs, read from its operand slots; kEdge: what lies
+// outside the image is zero.
+template <bool kEdge>
+__device__ __forceinline__ void mx_read(MxIn& in, lds_cu4* src, const MxGeo& G, int lane) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    in.a[b] = mx_op<kEdge>(src, b, 0, G, lane);
+    in.d[b] = mx_op<kEdge>(src, b, 1, G, lane);
+  }
+#pragma unroll
+  for (int ob = 0; ob < 2; ++ob) {
+    in.src[ob] = mx_px(src, ob, 0, G);
+    in.dis[ob] = mx_px(src, ob, 1, G);
+  }
+}
+
+// s_waitcnt immediate for lgkmcnt(0) only (vmcnt and expcnt at their maxima).
+constexpr int kLgkmZero = 15 | (7 << 4) | (0 << 8) | (3 << 14);
+
+// Workgroup barrier for data exchanged through ds_write / ds_read only: it
+// waits for the wave's LDS accesses but not for its DMA loads in flight
+// (which count in vmcnt; a release fence on LDS would wait for them too).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(kLgkmZero);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Sum over the four waves of block sum j, in wave order. The restrict
+// pointer tells the compiler's wait insertion that these reads do not touch
+// the DMA buffers.
+__device__ __forceinline__ double mx_red_sum(const __attribute__((address_space(3))) double* __restrict__ red, int j) {
+  double t = 0.0;
+#pragma unroll
+  for (int q = 0; q < kMxWaves; ++q) t += red[q * 18 + j];
+  return t;
+}
+
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_fast_tile_mfma(const PairDesc* __restrict__ pairs,
                                                              const int* __restrict__ prefix,
                                                              const int* __restrict__ meta) {
   static_assert(kThreads == 64 * kMxWaves, "four waves per block");
   __shared__ double red[kMxWaves][18];
+  __shared__ mx_u4 lbuf[kMxWaves][kMxOps * 64];
   const int npairs = meta[0], n = prefix[npairs];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1043,63 +1089,127 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
     K.v_lo[j] = lo;
   }
   ItemRange items(prefix, npairs, n);
-  for (int idx = items.first; idx < items.end; ++idx) {
-    const int pi = items.pair_of(idx);
-    const PairDesc& P = pairs[pi];
-    int s = 0, tile = idx - prefix[pi];
-    while (s + 1 < P.nscales && tile >= P.sc[s + 1].tile0) ++s;
-    tile -= P.sc[s].tile0;
-    const ScaleDesc& S = P.sc[s];
-    G.w = S.w;
-    G.h = S.h;
-    G.pitch = S.pitch;
-    const int x0 = (tile % S.tiles_x) * (kMxWaves * kMxCols);
-    G.y0 = (tile / S.tiles_x) * kMxRows;
-    G.xw = x0 + kMxCols * wv;
-    // every input window of the tile inside the image (so every output is
-    // 4 px or more from the border: b1 = 1, e = 0); uniform over the block
-    const bool interior = x0 >= 8 && x0 + kMxWaves * kMxCols + 8 <= G.w && G.y0 >= 4 && G.y0 + kMxRows + 4 <= G.h;
-    float acc[3][6];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int j = 0; j < 6; ++j) acc[c][j] = 0.0f;
-    // three channels, the next one's loads in flight while one is filtered
-    auto channels = [&](auto edge_tag) {
-      constexpr bool kEdge = decltype(edge_tag)::value;
-      MxIn in0, in1;
-      mx_load<kEdge>(in0, fast_plane<_Float16>(P, s, 0, 0), fast_plane<_Float16>(P, s, 1, 0), G);
-      mx_load<kEdge>(in1, fast_plane<_Float16>(P, s, 0, 1), fast_plane<_Float16>(P, s, 1, 1), G);
-      mx_channel<kEdge>(in0, K, xyb_shift(0), G, acc[0]);
-      mx_load<kEdge>(in0, fast_plane<_Float16>(P, s, 0, 2), fast_plane<_Float16>(P, s, 1, 2), G);
-      mx_channel<kEdge>(in1, K, xyb_shift(1), G, acc[1]);
-      mx_channel<kEdge>(in0, K, xyb_shift(2), G, acc[2]);
-    };
-    if (interior) {
-      channels(std::false_type{});
-    } else if (G.xw < G.w) {  // uniform per wave: a wave past the right edge adds zeros
-      mx_edge_setup(G);
-      channels(std::true_type{});
+  if (items.first >= items.end) return;  // uniform: a block beyond the last item
+  // A channel step: pair, scale, tile in the scale and the tile's geometry;
+  // interior = every input window inside the image (every output 4 px or
+  // more from the border: b1 = 1, e = 0), uniform over the block. Steps are
+  // kept in plain variables picked by selects, so they stay in scalar
+  // registers and the descriptors they address are read by scalar loads.
+  struct Step {
+    int pair, s, tile, w, h, y0, xw;
+    long long pitch;
+    int interior;
+  };
+  auto locate = [&](int idx) {
+    Step t;
+    t.pair = items.pair_of(idx);
+    // descriptors are constant over the launch: scalar loads
+    const CPairDesc& Q = ((const CPairDesc*)pairs)[t.pair];
+    int s = 0, tile = idx - prefix[t.pair];
+    while (s + 1 < Q.nscales && tile >= Q.sc[s + 1].tile0) ++s;
+    t.s = s;
+    t.tile = tile - Q.sc[s].tile0;
+    const int tiles_x = Q.sc[s].tiles_x;
+    t.w = Q.sc[s].w;
+    t.h = Q.sc[s].h;
+    t.pitch = Q.sc[s].pitch;
+    const int x0 = (t.tile % tiles_x) * (kMxWaves * kMxCols);
+    t.y0 = (t.tile / tiles_x) * kMxRows;
+    t.xw = x0 + kMxCols * wv;
+    t.interior = x0 >= 8 && x0 + kMxWaves * kMxCols + 8 <= t.w && t.y0 >= 4 && t.y0 + kMxRows + 4 <= t.h;
+    return t;
+  };
+  auto geo = [](MxGeo& g, const Step& t) {
+    g.w = t.w;
+    g.h = t.h;
+    g.pitch = t.pitch;
+    g.y0 = t.y0;
+    g.xw = t.xw;
+  };
+  lds_u4* const buf = (lds_u4*)&lbuf[wv][0];
+  // Step k is channel k % 3 of item first + k / 3. Its operands are read
+  // from the buffer into registers, then the DMA of step k + 1 refills the
+  // buffer while step k is filtered.
+  const int nsteps = 3 * (items.end - items.first);
+  int ld_idx = items.first, ld_c = 0;
+  Step ld = locate(ld_idx);
+  auto issue = [&]() {
+    // the step (ld_idx, ld_c) into the buffer; returns it and advances
+    const PairDesc& Q = pairs[ld.pair];
+    MxGeo gd = G;
+    geo(gd, ld);
+    mx_dma(buf, fast_plane<_Float16>(Q, ld.s, 0, ld_c), fast_plane<_Float16>(Q, ld.s, 1, ld_c), gd);
+    const Step t = ld;
+    if (++ld_c == 3 && ld_idx + 1 < items.end) {
+      ld_c = 0;
+      ld = locate(++ld_idx);
     }
+    return t;
+  };
+  Step nxt = issue();
+  float acc[3][6];
+  for (int k = 0; k < nsteps; ++k) {
+    const Step cur = nxt;
+    const int c = k % 3;
+    if (c == 0) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) mx_finish_sums(acc[c]);
-    // Block sums in a fixed order: f32 within each wave, then f64 over the
-    // four waves in sequence.
-    float v18[18];
+      for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < 6; ++j) acc[cc][j] = 0.0f;
+    }
+    geo(G, cur);
+    MxIn in;
+    if (cur.interior) {
+      mx_read<false>(in, buf, G, lane);
+    } else {
+      mx_edge_setup(G);
+      mx_read<true>(in, buf, G, lane);
+    }
+    // every read of the buffer has returned before the DMA refills it
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(kLgkmZero);
+    if (k + 1 < nsteps) nxt = issue();
+    auto channel = [&](auto c_tag) {
+      constexpr int C = decltype(c_tag)::value;
+      if (cur.interior)
+        mx_channel<false>(in, K, xyb_shift(C), G, acc[C]);
+      else if (G.xw < G.w)  // uniform per wave: a wave past the right edge adds zeros
+        mx_channel<true>(in, K, xyb_shift(C), G, acc[C]);
+    };
+    if (c == 0)
+      channel(std::integral_constant<int, 0>{});
+    else if (c == 1)
+      channel(std::integral_constant<int, 1>{});
+    else
+      channel(std::integral_constant<int, 2>{});
+    if (c == 2) {
+      const CPairDesc& P = ((const CPairDesc*)pairs)[cur.pair];
 #pragma unroll
-      for (int j = 0; j < 6; ++j) v18[c * 6 + j] = acc[c][j];
-    int which;
-    const float vs = wave_sums18(v18, lane, &which);
-    fast_barrier();  // the previous item's reads of red are done
-    if (which >= 0) red[wv][which] = (double)vs;
-    fast_barrier();
-    if (tid < 18) {
-      double t = 0.0;
-      for (int q = 0; q < kMxWaves; ++q) t += red[q][tid];
-      const int c = tid / 6, j = tid % 6;
-      ((__attribute__((address_space(1))) double*)P.partials)[(long long)(S.part + c * S.colblocks + tile) * kNumSums + j] = t;
+      for (int cc = 0; cc < 3; ++cc) mx_finish_sums(acc[cc]);
+      // Block sums in a fixed order: f32 within each wave, then f64 over the
+      // four waves in sequence.
+      float v18[18];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) v18[cc * 6 + j] = acc[cc][j];
+      int which;
+      const float vs = wave_sums18(v18, lane, &which);
+      lds_barrier();  // the previous item's reads of red are done
+      if (which >= 0) red[wv][which] = (double)vs;
+      lds_barrier();
+      if (tid < 18) {
+        // indexes of this lane's sum, derived here rather than held in
+        // registers across the loop
+        int i = tid;
+        asm volatile("" : "+v"(i));
+        const double t = mx_red_sum((const __attribute__((address_space(3))) double*)&red[0][0], i);
+        const int cc = i / 6, j = i % 6;
+        ((__attribute__((address_space(1))) double*)P.partials)[(long long)(P.sc[cur.s].part +
+                                                                            cc * P.sc[cur.s].colblocks + cur.tile) *
+                                                                    kNumSums +
+                                                                j] = t;
+      }
     }
   }
 }
