@@ -998,19 +998,6 @@ __device__ __forceinline__ void mx_dma(lds_u4* dst, const gT<_Float16>* pr, cons
   }
 }
 
-// Operand of row block b, image img, as the A operand of the horizontal pass.
-template <bool kEdge>
-__device__ __forceinline__ mx_h8 mx_op(lds_cu4* src, int b, int img, const MxGeo& G, int lane) {
-  mx_u4 u = src[(2 * b + img) * 64 + lane];
-  if constexpr (kEdge) {
-    const int yr = G.y0 - 4 + 16 * b + G.l, xc = G.xw - 8 + 8 * G.g;
-    const int nv = (yr >= 0 && yr < G.h && xc >= 0) ? G.w - xc : 0;  // columns inside the image
-#pragma unroll
-    for (int k = 0; k < 4; ++k) u[k] &= 2 * k + 1 < nv ? 0xFFFFFFFFu : (2 * k < nv ? 0x0000FFFFu : 0u);
-  }
-  return __builtin_bit_cast(mx_h8, u);
-}
-
 // Map pixels of output block ob (row y0 + 16 ob + l, columns xw + 4 g .. + 3)
 // of image img, from the operand slot that holds them.
 __device__ __forceinline__ mx_h4 mx_px(lds_cu4* src, int ob, int img, const MxGeo& G) {
@@ -1020,13 +1007,32 @@ __device__ __forceinline__ mx_h4 mx_px(lds_cu4* src, int ob, int img, const MxGe
 }
 
 // One channel step's inputs, read from its operand slots; kEdge: what lies
-// outside the image is zero.
+// outside the image is zero (the lane's window columns past the image by one
+// dword mask per step, rows outside the image as a whole).
 template <bool kEdge>
 __device__ __forceinline__ void mx_read(MxIn& in, lds_cu4* src, const MxGeo& G, int lane) {
+  unsigned cm[4];
+  if constexpr (kEdge) {
+    const int xc = G.xw - 8 + 8 * G.g;
+    const int nv = xc >= 0 ? G.w - xc : 0;  // columns of the window inside the image
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cm[k] = 2 * k + 1 < nv ? 0xFFFFFFFFu : (2 * k < nv ? 0x0000FFFFu : 0u);
+  }
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
-    in.a[b] = mx_op<kEdge>(src, b, 0, G, lane);
-    in.d[b] = mx_op<kEdge>(src, b, 1, G, lane);
+    mx_u4 ua = src[(2 * b) * 64 + lane], ud = src[(2 * b + 1) * 64 + lane];
+    if constexpr (kEdge) {
+      const int yr = G.y0 - 4 + 16 * b + G.l;
+      const bool rv = yr >= 0 && yr < G.h;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned m = rv ? cm[k] : 0u;
+        ua[k] &= m;
+        ud[k] &= m;
+      }
+    }
+    in.a[b] = __builtin_bit_cast(mx_h8, ua);
+    in.d[b] = __builtin_bit_cast(mx_h8, ud);
   }
 #pragma unroll
   for (int ob = 0; ob < 2; ++ob) {
@@ -1158,13 +1164,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
         for (int j = 0; j < 6; ++j) acc[cc][j] = 0.0f;
     }
     geo(G, cur);
+    if (c == 0 && !cur.interior) mx_edge_setup(G);  // kept for the item's three channels
     MxIn in;
-    if (cur.interior) {
+    if (cur.interior)
       mx_read<false>(in, buf, G, lane);
-    } else {
-      mx_edge_setup(G);
+    else
       mx_read<true>(in, buf, G, lane);
-    }
     // every read of the buffer has returned before the DMA refills it
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_waitcnt(kLgkmZero);
@@ -1195,6 +1200,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
         for (int j = 0; j < 6; ++j) v18[cc * 6 + j] = acc[cc][j];
       int which;
       const float vs = wave_sums18(v18, lane, &which);
+      asm volatile("" : "+v"(which));  // its address is formed here, not held across the loop
       lds_barrier();  // the previous item's reads of red are done
       if (which >= 0) red[wv][which] = (double)vs;
       lds_barrier();
