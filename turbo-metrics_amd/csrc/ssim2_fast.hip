@@ -991,8 +991,12 @@ __device__ __forceinline__ void mx_dma(lds_u4* dst, const gT<_Float16>* pr, cons
   const int xc = min(max(Gs.xw - 8 + 8 * Gs.g, 0), (int)Gs.pitch - 8);
 #pragma unroll
   for (int b = 0; b < 3; ++b) {
+    // rows past y0 + 35 carry weight 0: their slots keep the zeros written
+    // at kernel start
+    if (b == 2 && Gs.l >= 8) break;
     const int yr = min(max(Gs.y0 - 4 + 16 * b + Gs.l, 0), Gs.h - 1);
-    const long long o = (long long)yr * Gs.pitch + xc;
+    // sides are at most kMaxDim = 32768: a plane holds fewer than 2^32 elements
+    const unsigned o = (unsigned)yr * (unsigned)Gs.pitch + (unsigned)xc;
     __builtin_amdgcn_global_load_lds((gcvoid*)(pr + o), (void __attribute__((address_space(3)))*)(dst + (2 * b) * 64), 16, 0, 0);
     __builtin_amdgcn_global_load_lds((gcvoid*)(pd + o), (void __attribute__((address_space(3)))*)(dst + (2 * b + 1) * 64), 16, 0, 0);
   }
@@ -1139,6 +1143,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
   const int nsteps = 3 * (items.end - items.first);
   int ld_idx = items.first, ld_c = 0;
   Step ld = locate(ld_idx);
+  if (G.l >= 8) {  // the slots of row block 2 that no DMA writes (LDS starts undefined)
+    const mx_u4 z = {0u, 0u, 0u, 0u};
+    buf[4 * 64 + lane] = z;
+    buf[5 * 64 + lane] = z;
+  }
   auto issue = [&]() {
     // the step (ld_idx, ld_c) into the buffer; returns it and advances
     const PairDesc& Q = pairs[ld.pair];
@@ -1198,9 +1207,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) v
       for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
         for (int j = 0; j < 6; ++j) v18[cc * 6 + j] = acc[cc][j];
-      int which;
-      const float vs = wave_sums18(v18, lane, &which);
-      asm volatile("" : "+v"(which));  // its address is formed here, not held across the loop
+      // lane-derived shuffle addresses are formed here, not held across the loop
+      int which, ln = lane;
+      asm volatile("" : "+v"(ln));
+      const float vs = wave_sums18(v18, ln, &which);
       lds_barrier();  // the previous item's reads of red are done
       if (which >= 0) red[wv][which] = (double)vs;
       lds_barrier();
